@@ -1,0 +1,295 @@
+"""Benchmark of the CSR SpMM hot path on one MI355X.
+
+Headline workload (one step = one SpMM over the whole matrix, inputs resident
+in HBM): C = A B, A 262144 x 262144 CSR with power-law row lengths (exponent
+2.1, mean 24, cap 4096, distinct uniform columns, values U[-1,1]), B dense
+262144 x 64 N(0,1), float32 values, int32 indices, seed 1.
+
+value = algorithmic bytes / time per step (GB/s), where algorithmic bytes =
+nnz*(4+4) + (M+1)*4 + K*N*4 + M*N*4 (A streamed once, B read once, C written
+once).  The roofline object prices the dominant kernel (spmm_bins_kernel)
+against the 8 TB/s HBM peak with its duration from HIP events on the stream
+the kernels run on.  The same line also carries the bf16-vs-fp32 fwd+bwd
+speedup of the differentiable SpMM (200000-node power-law graph, N=128) and
+the 16384 x 16384 x 16 SpMV rate, and a CPU baseline: the C oracle (one core)
+on a bounded slice of the same workload, plus torch.sparse on all host cores.
+"""
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import flagsparse_amd as fs  # noqa: E402
+from flagsparse_amd import _plan, synthetic  # noqa: E402
+from flagsparse_amd._matrix import CsrHandle  # noqa: E402
+
+METRIC = "SpMV/SpMM GB/s (% HBM peak); bf16 fwd+bwd SpMM speedup vs FP32 at relerr<=1e-2"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def spmm_bytes(nnz, m, k, n, s_v=4, s_i=4, s_p=4, s_c=4):
+    return nnz * (s_v + s_i) + (m + 1) * s_p + k * n * s_v + m * n * s_c
+
+
+def time_steps(fn, steps, warmup, stream):
+    """Warm up, then time `steps` calls between device syncs; per-step HIP
+    event durations on `stream` (the stream the kernels are launched on)."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for s, e in events:
+        s.record(stream)
+        fn()
+        e.record(stream)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1000.0
+    ev = [s.elapsed_time(e) for s, e in events]
+    return wall_ms, ev
+
+
+def bench_spmm_power_law(dev, steps, warmup):
+    log("generating 262144^2 power-law CSR (seed 1)")
+    ip, ix, v, B = synthetic.power_law_rows(seed=1)
+    n = 262144
+    nnz = int(ix.size)
+    data, idx, ptr = synthetic.to_torch(ip, ix, v, dev, torch.float32, torch.int32)
+    Bt = synthetic.dense(B, dev, torch.float32)
+    handle = CsrHandle(idx, ptr, (n, n))
+    C = torch.empty((n, 64), dtype=torch.float32, device=dev)
+    lanes = _plan.lanes_for_dense(64, 4)
+    cfg = _plan.spmm_config(lanes)
+    plan = handle.plan(cfg)
+    stream = torch.cuda.current_stream(dev)
+    log(f"nnz={nnz} bins={plan.bin_counts} long_rows={plan.n_long_rows} segments={plan.n_segments}")
+
+    def step():
+        handle.spmm(data, Bt, out=C, lanes_n=lanes, config=cfg)
+
+    wall_ms, ev = time_steps(step, steps, warmup, stream)
+    ms = wall_ms / steps
+    nbytes = spmm_bytes(nnz, n, n, 64)
+    # trans on the cached CSC permutation (reported beside the headline)
+    Ct = torch.empty((n, 64), dtype=torch.float32, device=dev)
+    handle.csc()
+    handle.csc_plan(cfg)
+
+    def step_t():
+        handle.spmm(data, Bt, transpose=True, out=Ct, lanes_n=lanes, config=cfg)
+
+    wall_t, ev_t = time_steps(step_t, steps, warmup, stream)
+    ms_t = wall_t / steps
+    nbytes_t = nbytes + nnz * 4  # the permutation is read as well
+    return {
+        "nnz": nnz,
+        "ms": ms,
+        "event_ms_median": statistics.median(ev),
+        "bytes": nbytes,
+        "flops": 2.0 * nnz * 64,
+        "trans_ms": ms_t,
+        "trans_event_ms_median": statistics.median(ev_t),
+        "trans_bytes": nbytes_t,
+        "plan": {"bins": plan.bin_counts, "long_rows": plan.n_long_rows, "segments": plan.n_segments},
+        "inputs": (ip, ix, v, B),
+    }
+
+
+def bench_spmv_uniform(dev, steps, warmup):
+    ip, ix, v, x = synthetic.uniform_rows(16384, 16384, 16, seed=0)
+    data, idx, ptr = synthetic.to_torch(ip, ix, v, dev, torch.float32, torch.int32)
+    xt = synthetic.dense(x, dev, torch.float32)
+    handle = CsrHandle(idx, ptr, (16384, 16384))
+    y = torch.empty(16384, device=dev)
+    handle.spmv(data, xt, out=y)
+    stream = torch.cuda.current_stream(dev)
+    wall_ms, ev = time_steps(lambda: handle.spmv(data, xt, out=y), steps * 5, warmup, stream)
+    nbytes = ix.size * 8 + 16385 * 4 + 16384 * 4 * 2
+    med = statistics.median(ev)
+    return {"ms": wall_ms / (steps * 5), "event_ms_median": med, "bytes": nbytes,
+            "gbs_cache_resident": nbytes / (med * 1e-3) / 1e9}
+
+
+def bench_autograd_bf16_vs_fp32(dev, steps, warmup):
+    log("generating 200000-node power-law graph (seed 3)")
+    ip, ix, v = synthetic.power_law_graph(200000, 2.3, 14, seed=3)
+    n = 200000
+    rng = np.random.Generator(np.random.PCG64(3))
+    Bn = rng.standard_normal((n, 128))
+    dCn = rng.standard_normal((n, 128))
+    out = {"nnz": int(ix.size)}
+    results = {}
+    stream = torch.cuda.current_stream(dev)
+    for dtype, name in ((torch.float32, "fp32"), (torch.bfloat16, "bf16")):
+        data, idx, ptr = synthetic.to_torch(ip, ix, v, dev, dtype, torch.int32)
+        mat = fs.CsrMatrix(idx, ptr, (n, n)).prepare_backward()
+        vals = data.clone().requires_grad_(True)
+        B = synthetic.dense(Bn, dev, dtype).requires_grad_(True)
+        dC = synthetic.dense(dCn, dev, dtype)
+
+        def step():
+            vals.grad = None
+            B.grad = None
+            C = fs.spmm_csr_autograd(vals, B, mat)
+            C.backward(dC)
+
+        wall_ms, ev = time_steps(step, steps, warmup, stream)
+        out[name + "_ms"] = wall_ms / steps
+        out[name + "_event_ms_median"] = statistics.median(ev)
+        step()
+        results[name] = (fs.spmm_csr_autograd(vals, B, mat).detach().double(), B.grad.double(), vals.grad.double())
+    # bf16 error relative to the fp32 run (max|d| / max|ref| per output)
+    for i, what in enumerate(("fwd", "dB", "dvalues")):
+        ref = results["fp32"][i]
+        out[f"bf16_rel_err_{what}_vs_fp32"] = float((results["bf16"][i] - ref).abs().max() / ref.abs().max())
+    out["speedup_bf16_vs_fp32"] = out["fp32_ms"] / out["bf16_ms"]
+    return out
+
+
+def cpu_baseline(inputs, budget_s=20.0):
+    """C oracle (one core) on the first rows of the same workload, scaled to
+    GB/s with the same algorithmic-byte model; plus torch.sparse on all cores."""
+    from oracle import oracle  # noqa: PLC0415
+
+    ip, ix, v, B = inputs
+    n = ip.size - 1
+    rows = 8192
+    res = {}
+    while True:
+        sub_ip = ip[: rows + 1]
+        sub_nnz = int(sub_ip[-1])
+        t0 = time.perf_counter()
+        oracle.spmm(sub_ip, ix[:sub_nnz], v[:sub_nnz], B, (rows, n))
+        dt = time.perf_counter() - t0
+        if dt > 2.0 or rows >= n:
+            break
+        rows = min(n, rows * 4)
+    # the sample's share of the workload's algorithmic bytes (by nonzeros)
+    nbytes = spmm_bytes(int(ix.size), n, n, 64) * (sub_nnz / ix.size)
+    res["oracle"] = {"value": nbytes / dt / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
+                     "sample": f"C oracle (long double) SpMM over the first {rows} rows ({sub_nnz} nnz) of the "
+                               f"262144^2 power-law matrix, N=64, {dt:.2f} s"}
+    threads = os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    A = torch.sparse_csr_tensor(torch.from_numpy(ip), torch.from_numpy(ix), torch.from_numpy(v.astype(np.float32)),
+                                size=(n, n))
+    Bc = torch.from_numpy(B.astype(np.float32))
+    for _ in range(2):
+        torch.sparse.mm(A, Bc)
+    reps = []
+    t_start = time.perf_counter()
+    while len(reps) < 10 and time.perf_counter() - t_start < budget_s:
+        t0 = time.perf_counter()
+        torch.sparse.mm(A, Bc)
+        reps.append(time.perf_counter() - t0)
+    med = statistics.median(reps)
+    res["torch_sparse"] = {"value": spmm_bytes(int(ix.size), n, n, 64) / med / 1e9, "unit": "GB/s",
+                           "cores": torch.get_num_threads(), "ms": med * 1e3, "reps": len(reps),
+                           "cpu": _cpu_model()}
+    return res
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="headline SpMM only (profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the hot path is a single-GPU kernel); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+
+    r = bench_spmm_power_law(dev, args.steps, args.warmup)
+    value = r["bytes"] / (r["ms"] * 1e-3) / 1e9
+    kern_ms = r["event_ms_median"]
+    achieved = r["bytes"] / (kern_ms * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "csr_spmm_power_law_262144x262144_n64",
+            "op": "non",
+            "rows": 262144,
+            "cols": 262144,
+            "nnz": r["nnz"],
+            "dense_cols": 64,
+            "row_length": "power-law exponent 2.1, mean 24, cap 4096",
+            "value_dtype": "float32",
+            "index_dtype": "int32",
+            "seed": 1,
+        },
+        "hbm_frac": round(value / HBM_PEAK_GBS, 4),
+        "gflops": round(r["flops"] / (r["ms"] * 1e-3) / 1e9, 2),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "spmm_bins_kernel (+ spmm_long_reduce_kernel inside the event window)",
+            "kernel_ms": round(kern_ms, 5),
+        },
+        "plan": r["plan"],
+        "trans": {
+            "gbs": round(r["trans_bytes"] / (r["trans_ms"] * 1e-3) / 1e9, 2),
+            "ms_per_step": round(r["trans_ms"], 5),
+        },
+    }
+    if not args.no_extras:
+        s = bench_spmv_uniform(dev, args.steps, args.warmup)
+        line["spmv_uniform_16384x16"] = {"ms_per_step": round(s["ms"], 5),
+                                         "event_ms_median": round(s["event_ms_median"], 5),
+                                         "gbs_cache_resident": round(s["gbs_cache_resident"], 2)}
+        a = bench_autograd_bf16_vs_fp32(dev, max(5, args.steps // 5), args.warmup)
+        line["bf16_fwd_bwd_speedup_vs_fp32"] = round(a["speedup_bf16_vs_fp32"], 3)
+        line["autograd_200k"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in a.items()}
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        cb = cpu_baseline(r["inputs"])
+        line["cpu_baseline"] = cb["oracle"]
+        line["cpu_torch_sparse"] = cb["torch_sparse"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

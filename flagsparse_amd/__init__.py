@@ -1,0 +1,71 @@
+"""flagsparse_amd: MI355X-native (gfx950 HIP) CSR SpMV / SpMM / SDDMM.
+
+A drop-in for the CSR SpMV / SpMM (op non/trans/conj) / SDDMM path of
+FlagSparse: the public names below are those the reference exports for this
+path (src/flagsparse/__init__.py:19-154, conf/operators.yaml:46-61, 132-148,
+263-278), with the same arguments and behaviour.  Use it as
+``import flagsparse_amd as flagsparse``.  The compute is hand-written HIP for
+gfx950 in libflagsparse_hip.so; there is no other backend.
+"""
+
+from ._common import (
+    SUPPORTED_INDEX_DTYPES,
+    SUPPORTED_VALUE_DTYPES,
+    error_metrics,
+    validation_metrics,
+)
+from .autograd import CsrMatrix, flagsparse_spmm_csr_autograd, spmm_csr_autograd
+from .sddmm_csr import SDDMMPrepared, flagsparse_sddmm_csr, prepare_sddmm_csr
+from .spmm_csr import (
+    SPMM_CSR_ALGORITHMS,
+    PreparedCsrSpmmOpt,
+    PreparedCsrSpmmRoute,
+    SpmmCsrAlgorithm,
+    SpmmCsrAlgorithmUnavailable,
+    flagsparse_spmm_csr,
+    flagsparse_spmm_csr_opt,
+    flagsparse_spmm_csr_opt_alg1,
+    flagsparse_spmm_csr_opt_alg1_preprocess,
+    flagsparse_spmm_csr_run,
+    list_spmm_csr_algorithms,
+    prepare_spmm_csr_opt,
+    prepare_spmm_csr_opt_alg1,
+    prepare_spmm_csr_opt_alg1_preprocess,
+    prepare_spmm_csr_route,
+    resolve_spmm_csr_algorithm,
+)
+from .spmv_csr import PreparedCsrSpmv, flagsparse_spmv_csr, prepare_spmv_csr
+
+__version__ = "0.1.0"
+
+__all__ = [
+    "SUPPORTED_INDEX_DTYPES",
+    "SUPPORTED_VALUE_DTYPES",
+    "error_metrics",
+    "validation_metrics",
+    "PreparedCsrSpmv",
+    "flagsparse_spmv_csr",
+    "prepare_spmv_csr",
+    "SPMM_CSR_ALGORITHMS",
+    "PreparedCsrSpmmOpt",
+    "PreparedCsrSpmmRoute",
+    "SpmmCsrAlgorithm",
+    "SpmmCsrAlgorithmUnavailable",
+    "flagsparse_spmm_csr",
+    "flagsparse_spmm_csr_opt",
+    "flagsparse_spmm_csr_opt_alg1",
+    "flagsparse_spmm_csr_opt_alg1_preprocess",
+    "flagsparse_spmm_csr_run",
+    "list_spmm_csr_algorithms",
+    "prepare_spmm_csr_opt",
+    "prepare_spmm_csr_opt_alg1",
+    "prepare_spmm_csr_opt_alg1_preprocess",
+    "prepare_spmm_csr_route",
+    "resolve_spmm_csr_algorithm",
+    "SDDMMPrepared",
+    "flagsparse_sddmm_csr",
+    "prepare_sddmm_csr",
+    "CsrMatrix",
+    "flagsparse_spmm_csr_autograd",
+    "spmm_csr_autograd",
+]

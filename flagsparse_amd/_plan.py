@@ -1,0 +1,233 @@
+"""Row-length work plans for the CSR kernels (prepare-time, cached).
+
+A plan bins the rows of a CSR (or CSC) index structure by length on the GPU
+(``fsp_csr_row_bins``: a stable multi-split, rows stay ascending inside a
+bin) and cuts rows longer than the last bin into nnz-balanced segments.  The
+kernels then run every bin and every segment in one launch; rows that were
+split get a second, deterministic pass that adds their segment partials in
+order.  Building a plan costs one device->host copy of the bin boundaries and
+of the long rows' extents; running with a plan costs no host sync.
+
+Bin layouts per kernel family (``unroll`` = nonzeros per lane per trip):
+
+* SpMV: lanes per row L in {4, 8, 16, 32, 64}; a row of length n goes to the
+  smallest L with n <= L * unroll, L = 64 up to ``long_threshold``.
+* SpMM / SDDMM with G lanes along the dense dimension: nonzero slots per row
+  S in {1, 2, ..., 64 / G}; smallest S with n <= S * unroll, S = 64 / G up to
+  ``long_threshold``.
+
+The reference rebuilds its buckets with ``torch.nonzero`` on every call
+(spmv_csr.py:371-420, 910-911) or with atomic count / compact kernels
+(spmm_csr.py:3022-3072, 3114-3115).
+"""
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+
+# Defaults, overridable per call (the autotuner picks per shape class).
+SPMV_UNROLL = 4
+SPMV_LONG_THRESHOLD = 4096
+SPMV_SEG_LEN = 4096
+SPMM_UNROLL = 4
+SPMM_LONG_THRESHOLD = 256
+SPMM_SEG_LEN = 256
+SDDMM_UNROLL = 4
+SDDMM_LONG_THRESHOLD = 1024
+SDDMM_SEG_LEN = 1024
+
+
+@dataclass(frozen=True)
+class PlanConfig:
+    """Row-bin thresholds of one plan (hashable: it keys the plan cache)."""
+
+    lanes: tuple  # lanes (SpMV) or slots (SpMM/SDDMM) of each regular bin
+    uppers: tuple  # inclusive upper row length of each regular bin
+    seg_len: int  # segment length of rows above the last upper bound
+    unroll: int
+
+
+def spmv_config(unroll=SPMV_UNROLL, long_threshold=SPMV_LONG_THRESHOLD, seg_len=SPMV_SEG_LEN):
+    lanes = (4, 8, 16, 32, 64)
+    uppers = [min(l * unroll, long_threshold) for l in lanes[:-1]] + [long_threshold]
+    return _dedupe(lanes, uppers, seg_len, unroll)
+
+
+def rows_config(lanes_dense, unroll, long_threshold, seg_len):
+    """SpMM / SDDMM bins for `lanes_dense` lanes along the dense dimension."""
+    max_slots = max(1, 64 // int(lanes_dense))
+    lanes = []
+    s = 1
+    while s <= max_slots:
+        lanes.append(s)
+        s *= 2
+    uppers = [min(l * unroll, long_threshold) for l in lanes[:-1]] + [long_threshold]
+    return _dedupe(tuple(lanes), uppers, seg_len, unroll)
+
+
+def spmm_config(lanes_dense, unroll=SPMM_UNROLL, long_threshold=SPMM_LONG_THRESHOLD, seg_len=SPMM_SEG_LEN):
+    return rows_config(lanes_dense, unroll, long_threshold, seg_len)
+
+
+def sddmm_config(lanes_dense, unroll=SDDMM_UNROLL, long_threshold=SDDMM_LONG_THRESHOLD, seg_len=SDDMM_SEG_LEN):
+    return rows_config(lanes_dense, unroll, long_threshold, seg_len)
+
+
+def _dedupe(lanes, uppers, seg_len, unroll):
+    # Drop bins made empty by a small long_threshold (upper not increasing).
+    keep_l, keep_u = [], []
+    prev = -1
+    for l, u in zip(lanes, uppers):
+        if u > prev:
+            keep_l.append(l)
+            keep_u.append(u)
+            prev = u
+    return PlanConfig(tuple(keep_l), tuple(keep_u), int(max(1, seg_len)), int(unroll))
+
+
+def lanes_for_dense(n_dense, elem_bytes):
+    """Lanes along a dense dimension of n_dense elements read 16 bytes at a time."""
+    vec = max(1, 16 // int(elem_bytes))
+    vcols = -(-int(n_dense) // vec)
+    g = 1
+    while g < 64 and g < vcols:
+        g *= 2
+    return g
+
+
+def _index_code(t):
+    return _lib.index_code(t.dtype)
+
+
+@dataclass
+class CsrPlan:
+    """Device work plan of one index structure (see module docstring)."""
+
+    config: PlanConfig
+    n_rows: int
+    rows: torch.Tensor  # int32 [n_rows], regular bins then the long bin
+    bin_counts: list  # host counts of the regular bins
+    identity: bool  # the single regular bin holds rows 0..n_rows-1
+    seg_row: torch.Tensor
+    seg_begin: torch.Tensor
+    seg_end: torch.Tensor
+    long_rows: torch.Tensor
+    long_seg_ptr: torch.Tensor
+    n_segments: int
+    n_long_rows: int
+    _partials: dict = field(default_factory=dict)
+
+    @property
+    def bin_sizes(self):
+        return list(self.bin_counts) + [self.n_long_rows]
+
+    def partials(self, n_dense, acc_dtype, device):
+        """Scratch of n_segments x n_dense partial sums (cached per shape)."""
+        if self.n_segments == 0:
+            return None
+        key = (int(n_dense), acc_dtype)
+        buf = self._partials.get(key)
+        if buf is None:
+            buf = torch.empty(self.n_segments * max(1, int(n_dense)), dtype=acc_dtype, device=device)
+            self._partials[key] = buf
+        return buf
+
+    def struct(self, partials=None):
+        st = _lib.FspCsrPlan()
+        st.n_bins = len(self.bin_counts)
+        st.unroll = self.config.unroll
+        off = 0
+        for i, (lanes, count) in enumerate(zip(self.config.lanes, self.bin_counts)):
+            st.bins[i].lanes = int(lanes)
+            st.bins[i].count = int(count)
+            if self.identity or count == 0:
+                st.bins[i].rows = None
+            else:
+                st.bins[i].rows = self.rows.data_ptr() + 4 * off
+            off += count
+        st.n_segments = self.n_segments
+        st.n_long_rows = self.n_long_rows
+        if self.n_segments:
+            st.seg_row_dev = self.seg_row.data_ptr()
+            st.seg_begin_dev = self.seg_begin.data_ptr()
+            st.seg_end_dev = self.seg_end.data_ptr()
+            st.long_rows_dev = self.long_rows.data_ptr()
+            st.long_seg_ptr_dev = self.long_seg_ptr.data_ptr()
+        if partials is not None:
+            st.partials_dev = partials.data_ptr()
+            st.partials_bytes = partials.numel() * partials.element_size()
+        return st
+
+
+def build_plan(indptr, n_rows, config):
+    """Bin the rows described by `indptr` (device tensor, int32/int64)."""
+    device = indptr.device
+    n_rows = int(n_rows)
+    nb = len(config.uppers) + 1  # regular bins + the long bin
+    lib = _lib.load()
+    rows = torch.empty(max(1, n_rows), dtype=torch.int32, device=device)
+    offsets = torch.empty(nb + 1, dtype=torch.int64, device=device)
+    ws_bytes = ctypes.c_int64(0)
+    _lib.check(lib.fsp_csr_row_bins_workspace(n_rows, nb, ctypes.byref(ws_bytes)), "fsp_csr_row_bins_workspace")
+    ws = torch.empty(max(1, ws_bytes.value), dtype=torch.uint8, device=device)
+    uppers = (ctypes.c_int64 * nb)(*[int(u) for u in config.uppers], 0)
+    _lib.check(
+        lib.fsp_csr_row_bins(
+            _index_code(indptr), _lib.ptr(indptr), n_rows, nb, uppers, rows.data_ptr(),
+            offsets.data_ptr(), ws.data_ptr(), ws.numel(), _lib.stream_ptr(device),
+        ),
+        "fsp_csr_row_bins",
+    )
+    off = offsets.cpu().tolist()  # host sync (prepare time only)
+    counts = [off[i + 1] - off[i] for i in range(nb)]
+    regular = counts[:-1]
+    n_long = counts[-1]
+    identity = n_long == 0 and sum(1 for c in regular if c) <= 1
+    if identity and n_rows:
+        # one non-empty regular bin holding every row, in order: drop the list
+        regular = [n_rows if c else 0 for c in regular]
+    empty_i32 = torch.empty(0, dtype=torch.int32, device=device)
+    empty_i64 = torch.empty(0, dtype=torch.int64, device=device)
+    seg_row = seg_begin = seg_end = long_seg_ptr = empty_i64
+    long_rows = empty_i32
+    n_seg = 0
+    if n_long:
+        long_rows = rows[off[nb - 1]: off[nb]].clone()
+        lr = long_rows.to(torch.int64)
+        ip = indptr.to(torch.int64)
+        starts = ip[lr].cpu().numpy()
+        ends = ip[lr + 1].cpu().numpy()
+        lens = ends - starts
+        seg_len = config.seg_len
+        nseg = (lens + seg_len - 1) // seg_len
+        ptr = np.zeros(n_long + 1, dtype=np.int64)
+        np.cumsum(nseg, out=ptr[1:])
+        n_seg = int(ptr[-1])
+        owner = np.repeat(np.arange(n_long, dtype=np.int64), nseg)
+        j = np.arange(n_seg, dtype=np.int64) - ptr[owner]
+        # even split: segment j of a row covers [len*j/n, len*(j+1)/n)
+        b = starts[owner] + (lens[owner] * j) // nseg[owner]
+        e = starts[owner] + (lens[owner] * (j + 1)) // nseg[owner]
+        host_rows = long_rows.cpu().numpy().astype(np.int32)
+        seg_row = torch.from_numpy(host_rows[owner].copy()).to(device)
+        seg_begin = torch.from_numpy(b).to(device)
+        seg_end = torch.from_numpy(e).to(device)
+        long_seg_ptr = torch.from_numpy(ptr).to(device)
+    return CsrPlan(
+        config=config,
+        n_rows=n_rows,
+        rows=rows,
+        bin_counts=regular,
+        identity=bool(identity),
+        seg_row=seg_row if n_seg else empty_i32,
+        seg_begin=seg_begin,
+        seg_end=seg_end,
+        long_rows=long_rows,
+        long_seg_ptr=long_seg_ptr,
+        n_segments=n_seg,
+        n_long_rows=int(n_long),
+    )

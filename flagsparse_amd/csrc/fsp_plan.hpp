@@ -1,0 +1,147 @@
+// Launch table of the fused row-bin kernels.
+//
+// One launch covers every row bin of a plan plus the long-row segments: the
+// table (passed by value as a kernel argument) gives each bin its range of
+// workgroups; a workgroup finds its bin with a uniform scan of <= 9 entries.
+// Inside a bin, workgroup j processes row tiles j, j + nblocks, ... so the
+// grid stays bounded for any row count.
+#pragma once
+
+#include "fsp_device.hpp"
+#include "fsp_host.hpp"
+
+namespace fsp {
+
+constexpr int kMaxLaunchBins = FSP_MAX_BINS + 1;  // bins + the segment pseudo-bin
+constexpr int64_t kMaxBlocksPerBin = 1 << 20;
+
+struct launch_bin {
+  const int32_t* rows;   // NULL = identity (bins) ; unused for segments
+  int64_t count;         // rows (or segments) in the bin
+  int64_t block_begin;   // first workgroup of this bin
+  int64_t nblocks;       // workgroups of this bin
+  int32_t lanes_log2;    // SpMV: lanes per row; SpMM/SDDMM: nonzero slots per row
+  int32_t is_segment;    // 1 = items are long-row segments
+};
+
+struct launch_table {
+  launch_bin b[kMaxLaunchBins];
+  int32_t n;
+  int32_t pad;
+  const int32_t* seg_row;
+  const int64_t* seg_begin;
+  const int64_t* seg_end;
+};
+
+// Work item = (row, first nonzero, one past the last nonzero, output slot).
+struct work_item {
+  int64_t row;
+  int64_t begin;
+  int64_t end;
+  int64_t out;  // row for bins, segment index for segments
+};
+
+__device__ __forceinline__ int find_bin(const launch_table& t, int64_t block) {
+  int k = 0;
+  for (int i = 1; i < t.n; ++i) {
+    if (block >= t.b[i].block_begin) k = i;
+  }
+  return k;
+}
+
+template <typename PT>
+__device__ __forceinline__ work_item load_item(const launch_table& t, const launch_bin& lb,
+                                               const PT* __restrict__ indptr, int64_t rr) {
+  work_item w;
+  if (lb.is_segment) {
+    w.row = t.seg_row[rr];
+    w.begin = t.seg_begin[rr];
+    w.end = t.seg_end[rr];
+    w.out = rr;
+  } else {
+    w.row = lb.rows ? static_cast<int64_t>(lb.rows[rr]) : rr;
+    w.begin = static_cast<int64_t>(indptr[w.row]);
+    w.end = static_cast<int64_t>(indptr[w.row + 1]);
+    w.out = w.row;
+  }
+  return w;
+}
+
+static inline int ilog2(int64_t v) {
+  int l = 0;
+  while ((int64_t(1) << (l + 1)) <= v) ++l;
+  return l;
+}
+
+// Builds the launch table.  `items_per_block(lanes_log2, is_segment)` gives
+// the rows (segments) one workgroup covers per tile.  `seg_lanes_log2` is the
+// lanes_log2 used for segments.  Returns the total number of workgroups.
+template <typename F>
+static int64_t build_launch_table(const fsp_csr_plan* plan, int seg_lanes_log2, int max_lanes_log2,
+                                  F&& items_per_block, launch_table* t) {
+  int64_t blocks = 0;
+  t->n = 0;
+  for (int i = 0; i < plan->n_bins && i < FSP_MAX_BINS; ++i) {
+    const fsp_bin& b = plan->bins[i];
+    if (b.count <= 0) continue;
+    launch_bin& lb = t->b[t->n++];
+    int ll = ilog2(b.lanes > 0 ? b.lanes : 1);
+    if (ll > max_lanes_log2) ll = max_lanes_log2;
+    lb.rows = b.rows;
+    lb.count = b.count;
+    lb.lanes_log2 = ll;
+    lb.is_segment = 0;
+    const int64_t per = items_per_block(ll, 0);
+    int64_t nb = ceil_div(b.count, per);
+    if (nb > kMaxBlocksPerBin) nb = kMaxBlocksPerBin;
+    lb.block_begin = blocks;
+    lb.nblocks = nb;
+    blocks += nb;
+  }
+  if (plan->n_segments > 0) {
+    launch_bin& lb = t->b[t->n++];
+    lb.rows = nullptr;
+    lb.count = plan->n_segments;
+    lb.lanes_log2 = seg_lanes_log2;
+    lb.is_segment = 1;
+    const int64_t per = items_per_block(seg_lanes_log2, 1);
+    int64_t nb = ceil_div(plan->n_segments, per);
+    if (nb > kMaxBlocksPerBin) nb = kMaxBlocksPerBin;
+    lb.block_begin = blocks;
+    lb.nblocks = nb;
+    blocks += nb;
+  }
+  t->seg_row = plan->seg_row_dev;
+  t->seg_begin = plan->seg_begin_dev;
+  t->seg_end = plan->seg_end_dev;
+  return blocks;
+}
+
+static inline int check_plan(const fsp_csr_plan* plan, int64_t n_rows) {
+  if (plan == nullptr) return fail(FSP_ERR_ARG, "plan is NULL");
+  if (plan->n_bins < 0 || plan->n_bins > FSP_MAX_BINS) return fail(FSP_ERR_ARG, "plan: bad n_bins %d", plan->n_bins);
+  int64_t total = 0;
+  for (int i = 0; i < plan->n_bins; ++i) {
+    const fsp_bin& b = plan->bins[i];
+    if (b.count < 0 || b.lanes < 1 || b.lanes > 64 || !is_pow2(b.lanes)) {
+      return fail(FSP_ERR_ARG, "plan: bin %d has count %lld lanes %d", i, (long long)b.count, b.lanes);
+    }
+    if (b.rows == nullptr && b.count > n_rows) return fail(FSP_ERR_ARG, "plan: identity bin larger than n_rows");
+    total += b.count;
+  }
+  if (total + plan->n_long_rows != n_rows) {
+    return fail(FSP_ERR_ARG, "plan: bins cover %lld rows + %lld long rows, matrix has %lld",
+                (long long)total, (long long)plan->n_long_rows, (long long)n_rows);
+  }
+  if (plan->n_segments < 0 || plan->n_long_rows < 0) return fail(FSP_ERR_ARG, "plan: negative segment counts");
+  if (plan->n_segments > 0 && (plan->seg_row_dev == nullptr || plan->seg_begin_dev == nullptr ||
+                               plan->seg_end_dev == nullptr)) {
+    return fail(FSP_ERR_ARG, "plan: segment arrays missing");
+  }
+  if (plan->n_long_rows > 0 && (plan->long_rows_dev == nullptr || plan->long_seg_ptr_dev == nullptr)) {
+    return fail(FSP_ERR_ARG, "plan: long-row arrays missing");
+  }
+  return FSP_OK;
+}
+
+}  // namespace fsp

@@ -1,0 +1,156 @@
+// CSR SpMV y = A x on gfx950.
+//
+// Rows are binned by length; a bin with lanes L puts 64/L rows on one wave and
+// splits each row's nonzeros over L lanes (lane l takes nonzeros l, l+L, ...),
+// UNROLL independent (index, value, x) loads in flight per lane, then a DPP
+// group sum.  Rows longer than the last bin are cut into segments; one wave
+// reduces each segment into a partial and a second pass adds a row's partials
+// in segment order (no atomics: the result is deterministic for a plan).
+#include "fsp_device.hpp"
+#include "fsp_host.hpp"
+#include "fsp_plan.hpp"
+
+namespace fsp {
+
+template <typename T, typename IT, typename PT, typename QT, int UNROLL, bool PERM>
+__global__ __launch_bounds__(FSP_WG_THREADS) void spmv_bins_kernel(
+    const PT* __restrict__ indptr, const IT* __restrict__ indices, const T* __restrict__ data,
+    const QT* __restrict__ perm, const T* __restrict__ x, T* __restrict__ y,
+    typename spmv_acc_of<T>::type* __restrict__ partials, launch_table tab) {
+  using A = typename spmv_acc_of<T>::type;
+  const int k = find_bin(tab, blockIdx.x);
+  const launch_bin& lb = tab.b[k];
+  const int ll = lb.lanes_log2;
+  const int L = 1 << ll;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int sub = lane >> ll;
+  const int l = lane & (L - 1);
+  const int64_t per_wave = 64 >> ll;
+  const int64_t per_block = per_wave * FSP_WAVES;
+  for (int64_t tile = blockIdx.x - lb.block_begin; tile * per_block < lb.count; tile += lb.nblocks) {
+    const int64_t rr = tile * per_block + wave * per_wave + sub;
+    const bool active = rr < lb.count;
+    A acc = A(0);
+    work_item w{0, 0, 0, 0};
+    if (active) {
+      w = load_item(tab, lb, indptr, rr);
+      int64_t p = w.begin + l;
+      for (; p + (UNROLL - 1) * L < w.end; p += UNROLL * L) {
+        int64_t c[UNROLL];
+        T a[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+          const int64_t q = p + u * L;
+          c[u] = static_cast<int64_t>(indices[q]);
+          a[u] = data[PERM ? static_cast<int64_t>(perm[q]) : q];
+        }
+        T xv[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) xv[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) acc += static_cast<A>(a[u]) * static_cast<A>(xv[u]);
+      }
+      for (; p < w.end; p += L) {
+        const int64_t q = PERM ? static_cast<int64_t>(perm[p]) : p;
+        acc += static_cast<A>(data[q]) * static_cast<A>(x[static_cast<int64_t>(indices[p])]);
+      }
+    }
+    acc = group_sum(acc, L);
+    if (active && l == 0) {
+      if (lb.is_segment) {
+        partials[w.out] = acc;
+      } else {
+        y[w.out] = static_cast<T>(acc);
+      }
+    }
+  }
+}
+
+// y[long_rows[i]] = sum of the row's segment partials, in segment order.
+template <typename T>
+__global__ __launch_bounds__(FSP_WG_THREADS) void spmv_long_reduce_kernel(
+    const typename spmv_acc_of<T>::type* __restrict__ partials, const int32_t* __restrict__ long_rows,
+    const int64_t* __restrict__ seg_ptr, int64_t n_long, T* __restrict__ y) {
+  using A = typename spmv_acc_of<T>::type;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * FSP_WG_THREADS + threadIdx.x; i < n_long;
+       i += static_cast<int64_t>(gridDim.x) * FSP_WG_THREADS) {
+    A acc = A(0);
+    for (int64_t s = seg_ptr[i]; s < seg_ptr[i + 1]; ++s) acc += partials[s];
+    y[long_rows[i]] = static_cast<T>(acc);
+  }
+}
+
+template <typename T, typename IT, typename PT, typename QT, bool PERM>
+static int launch_spmv(const void* indptr, const void* indices, const void* data, const void* perm,
+                       const void* x, void* y, const fsp_csr_plan* plan, hipStream_t s) {
+  using A = typename spmv_acc_of<T>::type;
+  launch_table tab;
+  const int64_t blocks = build_launch_table(
+      plan, /*seg_lanes_log2=*/6, /*max_lanes_log2=*/6,
+      [](int ll, int) { return int64_t(FSP_WAVES) * (int64_t(64) >> ll); }, &tab);
+  if (plan->n_segments > 0 &&
+      (plan->partials_dev == nullptr || plan->partials_bytes < plan->n_segments * (int64_t)sizeof(A))) {
+    return fail(FSP_ERR_WORKSPACE, "fsp_spmv_csr: partials need %lld bytes",
+                (long long)(plan->n_segments * (int64_t)sizeof(A)));
+  }
+  A* partials = static_cast<A*>(plan->partials_dev);
+  if (blocks > 0) {
+    const int unroll = plan->unroll;
+#define FSP_SPMV_LAUNCH(U)                                                                          \
+  hipLaunchKernelGGL((spmv_bins_kernel<T, IT, PT, QT, U, PERM>), dim3(blocks), dim3(FSP_WG_THREADS), 0, s, \
+                     static_cast<const PT*>(indptr), static_cast<const IT*>(indices),                \
+                     static_cast<const T*>(data), static_cast<const QT*>(perm),                      \
+                     static_cast<const T*>(x), static_cast<T*>(y), partials, tab)
+    if (unroll <= 2) {
+      FSP_SPMV_LAUNCH(2);
+    } else if (unroll <= 4) {
+      FSP_SPMV_LAUNCH(4);
+    } else {
+      FSP_SPMV_LAUNCH(8);
+    }
+#undef FSP_SPMV_LAUNCH
+    const int st = check_launch("spmv_bins");
+    if (st) return st;
+  }
+  if (plan->n_long_rows > 0) {
+    int64_t g = ceil_div(plan->n_long_rows, FSP_WG_THREADS);
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(spmv_long_reduce_kernel<T>, dim3(g), dim3(FSP_WG_THREADS), 0, s, partials,
+                       plan->long_rows_dev, plan->long_seg_ptr_dev, plan->n_long_rows, static_cast<T*>(y));
+    return check_launch("spmv_long_reduce");
+  }
+  return FSP_OK;
+}
+
+}  // namespace fsp
+
+using namespace fsp;
+
+extern "C" int fsp_spmv_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
+                            const void* indptr_dev, const void* indices_dev, const void* data_dev,
+                            int32_t perm_type, const void* perm_dev, const void* x_dev, void* y_dev,
+                            int64_t n_rows, const fsp_csr_plan* plan, void* stream) {
+  if (n_rows < 0) return fail(FSP_ERR_ARG, "fsp_spmv_csr: negative n_rows");
+  int st = check_plan(plan, n_rows);
+  if (st) return st;
+  if (n_rows == 0) return FSP_OK;
+  if (indptr_dev == nullptr || y_dev == nullptr) return fail(FSP_ERR_ARG, "fsp_spmv_csr: NULL indptr or y");
+  const hipStream_t s = as_stream(stream);
+  const bool use_perm = perm_dev != nullptr;
+  FSP_DISPATCH_VALUE(value_type, T, {
+    FSP_DISPATCH_INDEX(index_type, IT, {
+      FSP_DISPATCH_INDEX(indptr_type, PT, {
+        if (!use_perm) {
+          return launch_spmv<T, IT, PT, int32_t, false>(indptr_dev, indices_dev, data_dev, nullptr, x_dev,
+                                                        y_dev, plan, s);
+        }
+        FSP_DISPATCH_INDEX(perm_type, QT, {
+          return launch_spmv<T, IT, PT, QT, true>(indptr_dev, indices_dev, data_dev, perm_dev, x_dev, y_dev,
+                                                  plan, s);
+        });
+      });
+    });
+  });
+  return FSP_OK;
+}

@@ -1,0 +1,150 @@
+"""Seeded synthetic CSR workloads (numpy PCG64; deterministic per seed).
+
+Matrices are canonical CSR: columns sorted and distinct inside each row.
+Arrays come back as numpy (indptr/indices int64, values float64); ``to_torch``
+moves them to a device in the requested dtypes.
+
+Workloads (sizes of the headline benchmarks):
+
+* ``uniform_rows``      -- every row has the same number of distinct uniform
+                           columns (SpMV 16384 x 16384, 16 per row).
+* ``power_law_rows``    -- row lengths n = min(cap, floor(x_m U^(-1/(a-1))))
+                           with U ~ U(0,1), exponent a, x_m solved so that
+                           E[n] = mean exactly (SpMM 262144 x 262144, a = 2.1,
+                           mean 24, cap 4096).
+* ``power_law_graph``   -- symmetric Chung-Lu graph (power-law weights, edges
+                           sampled proportionally to w_i w_j, symmetrised,
+                           deduplicated, no self loops), values
+                           D^-1/2 A D^-1/2 (200000 nodes, exponent 2.3,
+                           mean degree 14).
+"""
+
+import numpy as np
+import torch
+
+
+def _rng(seed):
+    return np.random.Generator(np.random.PCG64(int(seed)))
+
+
+def distinct_sorted_columns(rng, lengths, n_cols):
+    """Per-row distinct uniform columns in [0, n_cols), sorted inside rows."""
+    lengths = np.asarray(lengths, dtype=np.int64)
+    if np.any(lengths > n_cols):
+        raise ValueError("a row is longer than n_cols")
+    n_rows = lengths.size
+    indptr = np.zeros(n_rows + 1, dtype=np.int64)
+    np.cumsum(lengths, out=indptr[1:])
+    nnz = int(indptr[-1])
+    row_of = np.repeat(np.arange(n_rows, dtype=np.int64), lengths)
+    cols = rng.integers(0, n_cols, size=nnz, dtype=np.int64)
+    while True:
+        key = row_of * np.int64(n_cols) + cols
+        order = np.argsort(key, kind="stable")
+        key = key[order]
+        cols = cols[order]
+        dup = np.flatnonzero(key[1:] == key[:-1]) + 1
+        if dup.size == 0:
+            break
+        cols[dup] = rng.integers(0, n_cols, size=dup.size, dtype=np.int64)
+    return indptr, cols
+
+
+def uniform_rows(n_rows=16384, n_cols=16384, per_row=16, seed=0):
+    """(indptr, indices, values, x): values and x ~ U[-1, 1]."""
+    rng = _rng(seed)
+    indptr, indices = distinct_sorted_columns(rng, np.full(n_rows, per_row, dtype=np.int64), n_cols)
+    values = rng.uniform(-1.0, 1.0, size=indices.size)
+    x = rng.uniform(-1.0, 1.0, size=n_cols)
+    return indptr, indices, values, x
+
+
+def power_law_scale(alpha, mean, cap):
+    """x_m with sum_{k=1..cap} min(1, (x_m/k)^(alpha-1)) == mean (bisection)."""
+    k = np.arange(1, cap + 1, dtype=np.float64)
+
+    def expect(xm):
+        return float(np.minimum(1.0, (xm / k) ** (alpha - 1.0)).sum())
+
+    lo, hi = 1e-9, float(cap)
+    if expect(hi) < mean:
+        raise ValueError("mean not reachable under the cap")
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if expect(mid) < mean:
+            lo = mid
+        else:
+            hi = mid
+    return hi
+
+
+def power_law_lengths(rng, n_rows, alpha, mean, cap):
+    xm = power_law_scale(alpha, mean, cap)
+    u = 1.0 - rng.random(n_rows)  # (0, 1]
+    lengths = np.floor(xm * u ** (-1.0 / (alpha - 1.0)))
+    return np.clip(lengths, 0, cap).astype(np.int64)
+
+
+def power_law_rows(n_rows=262144, n_cols=262144, alpha=2.1, mean=24, cap=4096, n_dense=64, seed=1):
+    """(indptr, indices, values, B): values ~ U[-1, 1], B ~ N(0, 1) [n_cols, n_dense]."""
+    rng = _rng(seed)
+    lengths = power_law_lengths(rng, n_rows, alpha, mean, min(cap, n_cols))
+    indptr, indices = distinct_sorted_columns(rng, lengths, n_cols)
+    values = rng.uniform(-1.0, 1.0, size=indices.size)
+    B = rng.standard_normal((n_cols, n_dense)) if n_dense else None
+    return indptr, indices, values, B
+
+
+def power_law_graph(n_nodes=200000, alpha=2.3, mean_degree=14, seed=3):
+    """(indptr, indices, values) of D^-1/2 A D^-1/2 for a symmetric Chung-Lu graph."""
+    rng = _rng(seed)
+    u = 1.0 - rng.random(n_nodes)
+    w = u ** (-1.0 / (alpha - 1.0))
+    w = np.minimum(w, np.sqrt(w.sum()))  # Chung-Lu: keep w_i w_j / W <= 1
+    p = w / w.sum()
+    m = int(n_nodes * mean_degree // 2)
+    src = rng.choice(n_nodes, size=m, p=p)
+    dst = rng.choice(n_nodes, size=m, p=p)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    rows = np.concatenate([src, dst]).astype(np.int64)
+    cols = np.concatenate([dst, src]).astype(np.int64)
+    key = np.unique(rows * np.int64(n_nodes) + cols)
+    rows = key // n_nodes
+    cols = key % n_nodes
+    deg = np.bincount(rows, minlength=n_nodes).astype(np.int64)
+    indptr = np.zeros(n_nodes + 1, dtype=np.int64)
+    np.cumsum(deg, out=indptr[1:])
+    dinv = np.zeros(n_nodes)
+    nz = deg > 0
+    dinv[nz] = 1.0 / np.sqrt(deg[nz])
+    values = dinv[rows] * dinv[cols]
+    return indptr, cols, values
+
+
+def random_csr(n_rows, n_cols, density, seed, ensure_nonempty=True):
+    """Small test matrices: Bernoulli(density) pattern, values ~ N(0, 1) * 0.125."""
+    rng = _rng(seed)
+    mask = rng.random((n_rows, n_cols)) < density
+    if ensure_nonempty and n_rows and n_cols and not mask.any():
+        mask[0, 0] = True
+    rows, cols = np.nonzero(mask)
+    deg = np.bincount(rows, minlength=n_rows).astype(np.int64) if n_rows else np.zeros(0, np.int64)
+    indptr = np.zeros(n_rows + 1, dtype=np.int64)
+    np.cumsum(deg, out=indptr[1:])
+    values = rng.standard_normal(rows.size) * 0.125
+    return indptr, cols.astype(np.int64), values
+
+
+def to_torch(indptr, indices, values, device, value_dtype=torch.float32, index_dtype=torch.int32,
+             indptr_dtype=None):
+    indptr_dtype = indptr_dtype or index_dtype
+    return (
+        torch.from_numpy(np.ascontiguousarray(values)).to(device=device, dtype=value_dtype),
+        torch.from_numpy(np.ascontiguousarray(indices)).to(device=device, dtype=index_dtype),
+        torch.from_numpy(np.ascontiguousarray(indptr)).to(device=device, dtype=indptr_dtype),
+    )
+
+
+def dense(array, device, dtype=torch.float32):
+    return torch.from_numpy(np.ascontiguousarray(array)).to(device=device, dtype=dtype)

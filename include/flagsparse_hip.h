@@ -1,0 +1,166 @@
+/*
+ * flagsparse_hip.h -- C ABI of libflagsparse_hip.so, the gfx950 (MI355X) kernels
+ * behind the flagsparse_amd CSR SpMV / SpMM / SDDMM operators.
+ *
+ * Conventions
+ *   - Every pointer named *_dev is device memory; everything else is host memory.
+ *   - Sizes are int64_t.  Value / index types are the FSP_* enums below.
+ *   - Every entry point takes the hipStream_t to launch on (passed as void*;
+ *     NULL = the default stream), is re-entrant, never allocates device memory
+ *     and never synchronises the device unless its comment says so.  Scratch
+ *     memory is always supplied by the caller (sizes come from *_workspace()).
+ *   - Return value: FSP_OK (0) or a negative fsp_status; fsp_last_error()
+ *     returns a thread-local message for the last failure on this thread.
+ *
+ * The reference (FlagSparse) is pure Python + Triton: its hot path has no
+ * native FFI.  Each entry point below replaces the Triton launch site named in
+ * its comment (file:line relative to src/flagsparse/sparse_operations/).
+ */
+#ifndef FLAGSPARSE_HIP_H
+#define FLAGSPARSE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSP_ABI_VERSION 1
+#define FSP_MAX_BINS 8
+
+typedef enum fsp_value_type {
+  FSP_F32 = 0,
+  FSP_F64 = 1,
+  FSP_F16 = 2,
+  FSP_BF16 = 3,
+  FSP_F8E4M3 = 4 /* OCP e4m3 (finite-only, bias 7); BSR SpMM operand only */
+} fsp_value_type;
+
+typedef enum fsp_index_type { FSP_I32 = 0, FSP_I64 = 1 } fsp_index_type;
+
+typedef enum fsp_status {
+  FSP_OK = 0,
+  FSP_ERR_ARG = -1,       /* bad size / pointer / combination */
+  FSP_ERR_DTYPE = -2,     /* value or index type not supported by this entry */
+  FSP_ERR_LAUNCH = -3,    /* the HIP runtime rejected a launch or copy */
+  FSP_ERR_WORKSPACE = -4  /* caller-supplied scratch too small */
+} fsp_status;
+
+/* One row-length bin of a CSR work plan.  `lanes` is the number of lanes that
+ * cooperate on one row: SpMV splits a row's nonzeros over `lanes` lanes; SpMM
+ * and SDDMM split them over `lanes` nonzero slots (each slot spans the lanes
+ * that cover the dense dimension). */
+typedef struct fsp_bin {
+  int32_t lanes;
+  int32_t reserved;
+  int64_t count;         /* rows in this bin */
+  const int32_t* rows;   /* device row ids, ascending; NULL = rows 0..count-1 */
+} fsp_bin;
+
+/* Work plan of one CSR (or CSC) matrix: rows binned by length, plus the
+ * nnz-balanced split of rows longer than the last bin into segments.  All
+ * arrays are device memory owned by the caller (built by fsp_csr_row_bins and
+ * the host-side segment split). */
+typedef struct fsp_csr_plan {
+  int32_t n_bins;
+  int32_t unroll;                 /* nonzeros per lane per loop trip (1..8) */
+  fsp_bin bins[FSP_MAX_BINS];
+  int64_t n_segments;             /* long-row segments */
+  const int32_t* seg_row_dev;     /* [n_segments] row of each segment */
+  const int64_t* seg_begin_dev;   /* [n_segments] first nonzero */
+  const int64_t* seg_end_dev;     /* [n_segments] one past the last nonzero */
+  int64_t n_long_rows;
+  const int32_t* long_rows_dev;   /* [n_long_rows] */
+  const int64_t* long_seg_ptr_dev;/* [n_long_rows+1] segment range per long row */
+  void* partials_dev;             /* scratch: n_segments * dense_cols * acc bytes */
+  int64_t partials_bytes;
+} fsp_csr_plan;
+
+/* ---- library ------------------------------------------------------------ */
+int fsp_abi_version(void);
+const char* fsp_last_error(void);
+/* gcnArchName of the current device into buf (e.g. "gfx950:sramecc+:xnack-"). */
+int fsp_device_arch(char* buf, int64_t buf_len);
+
+/* ---- plan building -------------------------------------------------------
+ * Replaces the per-call torch.nonzero row bucketing of
+ * spmv_csr.py:371-420 (_build_spmv_opt_buckets) and the symbolic count /
+ * compact kernels of spmm_csr.py:3022-3072.  Rows are stably partitioned by
+ * length into n_bins bins: bin b holds rows with
+ * upper[b-1] < len <= upper[b] (upper[-1] = -1, upper[n_bins-1] = +inf).
+ * rows_out_dev[n_rows] receives the row ids bin after bin (ascending inside
+ * each bin); bin_offsets_out_dev[n_bins+1] the bin boundaries. */
+int fsp_csr_row_bins_workspace(int64_t n_rows, int32_t n_bins, int64_t* bytes);
+int fsp_csr_row_bins(int32_t indptr_type, const void* indptr_dev, int64_t n_rows,
+                     int32_t n_bins, const int64_t* upper,
+                     int32_t* rows_out_dev, int64_t* bin_offsets_out_dev,
+                     void* ws_dev, int64_t ws_bytes, void* stream);
+
+/* ---- CSR -> CSC -----------------------------------------------------------
+ * Replaces _transpose_csr_for_spmv (spmv_csr.py:498-536) and
+ * _transpose_csr_for_spmm (spmm_csr.py:407-445): a stable bucketing of the
+ * nonzeros by column.  Outputs colptr[n_cols+1] (type colptr_type),
+ * rowind[nnz] (type rowind_type) and perm[nnz] (type perm_type) with
+ * CSC position q holding CSR nonzero perm[q]; entries of one column keep CSR
+ * order (ascending row).  Bit-exact with a stable argsort by column. */
+int fsp_csr_to_csc_workspace(int64_t n_rows, int64_t n_cols, int64_t nnz,
+                             int32_t perm_type, int64_t* bytes);
+int fsp_csr_to_csc(int32_t indptr_type, int32_t index_type,
+                   const void* indptr_dev, const void* indices_dev,
+                   int64_t n_rows, int64_t n_cols, int64_t nnz,
+                   int32_t colptr_type, void* colptr_dev,
+                   int32_t rowind_type, void* rowind_dev,
+                   int32_t perm_type, void* perm_dev,
+                   void* ws_dev, int64_t ws_bytes, void* stream);
+/* out[q] = in[perm[q]] for q < nnz (value bytes = elem_bytes: 1,2,4,8,16). */
+int fsp_gather(int32_t elem_bytes, int32_t perm_type, const void* perm_dev,
+               const void* in_dev, void* out_dev, int64_t nnz, void* stream);
+
+/* ---- SpMV ----------------------------------------------------------------
+ * y = A x over a CSR (op non) or over the CSC of A (op trans/conj; pass the
+ * CSC arrays and perm != NULL so values are read as data[perm[q]]).
+ * Replaces _spmv_csr_real_kernel (spmv_csr.py:179-205) as launched by
+ * _triton_spmv_csr_impl_prepared (spmv_csr.py:702-746) and the bucketed
+ * kernels of the use_opt path (spmv_csr.py:247-368, 423-482).
+ * Accumulation: f64 for f32/f64 values (the reference promotes f32 SpMV to
+ * f64, spmv_csr.py:138-143), f32 for f16/bf16.  y has value_type. */
+int fsp_spmv_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
+                 const void* indptr_dev, const void* indices_dev,
+                 const void* data_dev, int32_t perm_type, const void* perm_dev,
+                 const void* x_dev, void* y_dev, int64_t n_rows,
+                 const fsp_csr_plan* plan, void* stream);
+
+/* ---- SpMM ----------------------------------------------------------------
+ * C[n_rows, n_dense] = A B with B row-major (ldb >= n_dense) and C row-major
+ * (ldc >= n_dense).  Same CSR / CSC+perm convention as fsp_spmv_csr.
+ * Replaces _spmm_csr_real_kernel (spmm_csr.py:144-200) launched by
+ * _triton_spmm_csr_impl (spmm_csr.py:3373-3466), and the bucketed /
+ * long-row split kernels of the opt path (spmm_csr.py:3469-3794).
+ * Accumulation: f32 for f32/f16/bf16, f64 for f64 (spmm_csr.py:3419).
+ * `lanes_n` = lanes spanning the dense dimension (power of two, 1..64). */
+int fsp_spmm_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
+                 const void* indptr_dev, const void* indices_dev,
+                 const void* data_dev, int32_t perm_type, const void* perm_dev,
+                 const void* B_dev, int64_t ldb, void* C_dev, int64_t ldc,
+                 int64_t n_rows, int64_t n_dense, int32_t lanes_n,
+                 const fsp_csr_plan* plan, void* stream);
+
+/* ---- SDDMM ---------------------------------------------------------------
+ * out[p] = alpha * <X[row(p), :], Y[col(p), :]> + beta * data_in[p]
+ * (data_in may be NULL when beta == 0).  X is [n_rows, k] (row stride ldx),
+ * Y is [n_cols, k] (row stride ldy).  Replaces _sddmm_csr_real_kernel
+ * (sddmm_csr.py:121-164) launched by _run_sddmm_prepared (sddmm_csr.py:291-386).
+ * Accumulation f32 (f32/f16/bf16) or f64 (f64).  The plan's segments cut long
+ * rows; SDDMM needs no partials. */
+int fsp_sddmm_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
+                  const void* indptr_dev, const void* indices_dev,
+                  const void* X_dev, int64_t ldx, const void* Y_dev, int64_t ldy,
+                  const void* data_in_dev, void* out_dev, int64_t n_rows,
+                  int64_t k, double alpha, double beta, int32_t lanes_k,
+                  const fsp_csr_plan* plan, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLAGSPARSE_HIP_H */

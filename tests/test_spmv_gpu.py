@@ -1,0 +1,233 @@
+"""SpMV parity on the GPU: HIP kernels vs the C oracle (FP64 golden).
+
+Mirrors the reference's tests/pytest/test_spmv_csr_accuracy.py grid
+(shapes (1,32), (160,1024), (128,256); int32/int64; non/trans/conj;
+prepared mismatch errors; int64 -> int32 fallback policy) with fixed seeds,
+plus the 16384 x 16384 x 16 workload, long rows (segment split), empty
+rows/matrices and bitwise determinism.  Bound: max|d|/max|ref| <= 1e-5 (f32),
+1e-12 (f64), 1e-2 (bf16), 5e-3 (f16).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import flagsparse_amd as fs
+from flagsparse_amd import spmv_csr as spmv_mod
+from flagsparse_amd import synthetic
+from oracle import oracle
+
+from ._helpers import assert_rel, device_csr, long_row_matrix, np_dtype_round
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(1, 32), (160, 1024), (128, 256)]
+
+
+@pytest.mark.parametrize("M,N", SHAPES)
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["float32", "float64"])
+@pytest.mark.parametrize("index_dtype", [torch.int32, torch.int64], ids=["int32", "int64"])
+@pytest.mark.parametrize("op", ["non", "trans", "conj"])
+def test_spmv_matches_oracle(device, M, N, dtype, index_dtype, op):
+    ip, ix, v = synthetic.random_csr(M, N, min(0.25, max(0.06, 32.0 / (M * N))), seed=M * 1000 + N)
+    data, idx, ptr = device_csr(ip, ix, v, device, dtype, index_dtype)
+    x_len = M if op != "non" else N
+    x = np.random.default_rng(M + N).standard_normal(x_len)
+    xt = torch.tensor(x, dtype=dtype, device=device)
+    y = fs.flagsparse_spmv_csr(data, idx, ptr, xt, shape=(M, N), op=op, index_fallback_policy="auto")
+    assert y.dtype == dtype and y.shape == (N if op != "non" else M,)
+    ref = oracle.spmv(ip, ix, np_dtype_round(v, dtype), np_dtype_round(x, dtype), (M, N), op)
+    assert_rel(y, ref, dtype, f"spmv {op}")
+
+
+@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16], ids=["bfloat16", "float16"])
+@pytest.mark.parametrize("op", ["non", "trans"])
+def test_spmv_half_storage(device, dtype, op):
+    ip, ix, v = synthetic.random_csr(300, 200, 0.1, seed=7)
+    data, idx, ptr = device_csr(ip, ix, v, device, dtype)
+    x = np.random.default_rng(7).standard_normal(300 if op == "trans" else 200)
+    y = fs.flagsparse_spmv_csr(data, idx, ptr, torch.tensor(x, dtype=dtype, device=device), shape=(300, 200), op=op)
+    ref = oracle.spmv(ip, ix, np_dtype_round(v, dtype), np_dtype_round(x, dtype), (300, 200), op)
+    assert_rel(y, ref, dtype, "spmv half")
+
+
+def test_spmv_uniform_16384x16_full_size(device):
+    ip, ix, v, x = synthetic.uniform_rows(16384, 16384, 16, seed=0)
+    data, idx, ptr = device_csr(ip, ix, v, device, torch.float32)
+    prepared = fs.prepare_spmv_csr(data, idx, ptr, (16384, 16384))
+    y = fs.flagsparse_spmv_csr(x=torch.tensor(x, dtype=torch.float32, device=device), prepared=prepared)
+    ref = oracle.spmv(ip, ix, np_dtype_round(v, torch.float32), np_dtype_round(x, torch.float32), (16384, 16384))
+    err = assert_rel(y, ref, torch.float32, "uniform 16x")
+    # f64 accumulation + one rounding: every entry equals the correctly rounded oracle or is 1 ulp off
+    y64 = y.double().cpu().numpy()
+    assert np.max(np.abs(y64 - ref.astype(np.float32).astype(np.float64))) <= 1.2e-7 * max(1.0, np.abs(ref).max())
+    assert err < 1e-6
+
+
+@pytest.mark.parametrize("long_len", [4097, 9000, 40000])
+@pytest.mark.parametrize("op", ["non", "trans"])
+def test_spmv_long_rows_split_into_segments(device, long_len, op):
+    n_cols = 50000
+    ip, ix, v = long_row_matrix(64, n_cols, [0, 5, 63], long_len, 9, seed=long_len)
+    data, idx, ptr = device_csr(ip, ix, v, device, torch.float64, torch.int64)
+    prepared = fs.prepare_spmv_csr(data, idx, ptr, (64, n_cols), op=op)
+    x = np.random.default_rng(1).standard_normal(64 if op == "trans" else n_cols)
+    y = fs.flagsparse_spmv_csr(x=torch.tensor(x, device=device), prepared=prepared, op=op)
+    ref = oracle.spmv(ip, ix, v, x, (64, n_cols), op)
+    assert_rel(y, ref, torch.float64, "long rows")
+    if op == "non":
+        assert prepared.plan().n_long_rows == 3 and prepared.plan().n_segments >= 3
+
+
+def test_spmv_all_row_bins_and_empty_rows(device):
+    # lengths 0..300 cover every lane bin; zero-length rows must produce 0
+    rng = np.random.Generator(np.random.PCG64(5))
+    lengths = np.concatenate([np.zeros(50, np.int64), rng.integers(0, 300, 2000)])
+    rng.shuffle(lengths)
+    ip, ix = synthetic.distinct_sorted_columns(rng, lengths, 1000)
+    v = rng.uniform(-1, 1, ix.size)
+    data, idx, ptr = device_csr(ip, ix, v, device, torch.float32)
+    x = rng.uniform(-1, 1, 1000)
+    y = fs.flagsparse_spmv_csr(data, idx, ptr, torch.tensor(x, dtype=torch.float32, device=device),
+                               shape=(lengths.size, 1000))
+    ref = oracle.spmv(ip, ix, np_dtype_round(v, torch.float32), np_dtype_round(x, torch.float32),
+                      (lengths.size, 1000))
+    assert_rel(y, ref, torch.float32, "bins")
+    assert torch.all(y[torch.from_numpy(lengths == 0).to(device)] == 0)
+
+
+def test_spmv_empty_matrix_and_zero_rows(device):
+    ptr = torch.zeros(5, dtype=torch.int32, device=device)
+    idx = torch.empty(0, dtype=torch.int32, device=device)
+    data = torch.empty(0, dtype=torch.float32, device=device)
+    y = fs.flagsparse_spmv_csr(data, idx, ptr, torch.ones(7, device=device), shape=(4, 7))
+    assert torch.equal(y, torch.zeros(4, device=device))
+    y = fs.flagsparse_spmv_csr(data, idx, ptr, torch.ones(4, device=device), shape=(4, 7), op="trans")
+    assert torch.equal(y, torch.zeros(7, device=device))
+    y = fs.flagsparse_spmv_csr(data, idx, torch.zeros(1, dtype=torch.int32, device=device),
+                               torch.ones(3, device=device), shape=(0, 3))
+    assert y.shape == (0,)
+
+
+def test_spmv_deterministic_bitwise(device):
+    ip, ix, v, _ = synthetic.power_law_rows(20000, 20000, 2.1, 24, 4096, n_dense=0, seed=3)
+    data, idx, ptr = device_csr(ip, ix, v, device, torch.float32)
+    prepared = fs.prepare_spmv_csr(data, idx, ptr, (20000, 20000))
+    x = torch.randn(20000, device=device)
+    y1 = fs.flagsparse_spmv_csr(x=x, prepared=prepared)
+    y2 = fs.flagsparse_spmv_csr(x=x, prepared=prepared)
+    assert torch.equal(y1, y2)
+
+
+def test_spmv_out_argument(device):
+    ip, ix, v = synthetic.random_csr(40, 30, 0.2, seed=2)
+    data, idx, ptr = device_csr(ip, ix, v, device, torch.float32)
+    x = torch.randn(30, device=device)
+    out = torch.empty(40, device=device)
+    y = fs.flagsparse_spmv_csr(data, idx, ptr, x, (40, 30), out=out)
+    assert y is out
+    with pytest.raises(ValueError, match="out shape/dtype must match result"):
+        fs.flagsparse_spmv_csr(data, idx, ptr, x, (40, 30), out=torch.empty(41, device=device))
+
+
+def test_spmv_return_time_and_meta(device):
+    ip, ix, v = synthetic.random_csr(40, 30, 0.2, seed=2)
+    data, idx, ptr = device_csr(ip, ix, v, device, torch.float32)
+    x = torch.randn(30, device=device)
+    y, ms, meta = fs.flagsparse_spmv_csr(data, idx, ptr, x, (40, 30), return_time=True, return_meta=True,
+                                         use_opt=True)
+    assert ms == pytest.approx(meta["op_total_ms"]) and meta["bucket_count"] >= 1
+    assert set(meta) == {"symbolic_ms", "compute_ms", "op_total_ms", "bucket_count"}
+
+
+def test_spmv_prepared_transpose_mismatch_rejected(device):
+    ip, ix, v = synthetic.random_csr(8, 10, 0.3, seed=1)
+    data, idx, ptr = device_csr(ip, ix, v, device, torch.float32)
+    prepared = spmv_mod.prepare_spmv_csr(data, idx, ptr, (8, 10), transpose=True)
+    x = torch.randn(8, device=device)
+    with pytest.raises(ValueError, match="does not match prepared.transpose"):
+        fs.flagsparse_spmv_csr(x=x, prepared=prepared, transpose=False)
+
+
+def test_spmv_prepared_op_mismatch_rejected(device):
+    ip, ix, v = synthetic.random_csr(8, 10, 0.3, seed=1)
+    data, idx, ptr = device_csr(ip, ix, v, device, torch.float32)
+    prepared = spmv_mod.prepare_spmv_csr(data, idx, ptr, (8, 10), op="conj")
+    with pytest.raises(ValueError, match="does not match prepared.op"):
+        fs.flagsparse_spmv_csr(x=torch.randn(8, device=device), prepared=prepared, op="trans")
+
+
+def test_spmv_validation_errors_on_device(device):
+    ip, ix, v = synthetic.random_csr(8, 10, 0.3, seed=1)
+    data, idx, ptr = device_csr(ip, ix, v, device, torch.float32)
+    with pytest.raises(ValueError, match="indptr must start at zero"):
+        fs.flagsparse_spmv_csr(data, idx, ptr + 1, torch.randn(10, device=device), (8, 10))
+    bad = idx.clone()
+    bad[0] = 10
+    with pytest.raises(IndexError, match="indices out of range for n_cols"):
+        fs.flagsparse_spmv_csr(data, bad, ptr, torch.randn(10, device=device), (8, 10))
+    with pytest.raises(TypeError, match="x dtype must match"):
+        fs.flagsparse_spmv_csr(data, idx, ptr, torch.randn(10, device=device, dtype=torch.float64), (8, 10))
+    with pytest.raises(ValueError, match="x length must be n_cols=10, got 9"):
+        fs.flagsparse_spmv_csr(data, idx, ptr, torch.randn(9, device=device), (8, 10))
+
+
+def test_spmv_int64_auto_fallback_to_int32(device, monkeypatch):
+    ip, ix, v = synthetic.random_csr(12, 9, 0.3, seed=4)
+    data, idx, ptr = device_csr(ip, ix, v, device, torch.float32, torch.int64)
+    x = np.random.default_rng(4).standard_normal(9)
+    state = {"forced_once": False}
+    original = spmv_mod._hip_spmv_csr_impl_prepared
+
+    def fail_int64_once(prepared, x_in, out=None):
+        if prepared.kernel_indices.dtype == torch.int64 and not state["forced_once"]:
+            state["forced_once"] = True
+            raise RuntimeError("forced int64 launch failure")
+        return original(prepared, x_in, out)
+
+    monkeypatch.setattr(spmv_mod, "_hip_spmv_csr_impl_prepared", fail_int64_once)
+    y = fs.flagsparse_spmv_csr(data, idx, ptr, torch.tensor(x, dtype=torch.float32, device=device), shape=(12, 9),
+                               index_fallback_policy="auto")
+    assert state["forced_once"]
+    assert_rel(y, oracle.spmv(ip, ix, np_dtype_round(v, torch.float32), np_dtype_round(x, torch.float32), (12, 9)),
+               torch.float32)
+
+
+def test_spmv_int64_strict_no_fallback(device, monkeypatch):
+    ip, ix, v = synthetic.random_csr(12, 9, 0.3, seed=4)
+    data, idx, ptr = device_csr(ip, ix, v, device, torch.float32, torch.int64)
+
+    def fail_int64(prepared, x_in, out=None):
+        raise RuntimeError("forced int64 launch failure")
+
+    monkeypatch.setattr(spmv_mod, "_hip_spmv_csr_impl_prepared", fail_int64)
+    with pytest.raises(RuntimeError, match="forced int64 launch failure"):
+        fs.flagsparse_spmv_csr(data, idx, ptr, torch.randn(9, device=device), shape=(12, 9),
+                               index_fallback_policy="strict")
+
+
+def test_spmv_int64_auto_does_not_fallback_when_index_exceeds_int32(device, monkeypatch):
+    limit = spmv_mod._INDEX_LIMIT_INT32
+    data = torch.ones(1, dtype=torch.float32, device=device)
+    prepared = spmv_mod.PreparedCsrSpmv(
+        data=data,
+        kernel_indices=torch.tensor([limit + 1], dtype=torch.int64, device=device),
+        kernel_indptr=torch.tensor([0, 1], dtype=torch.int64, device=device),
+        shape=(1, limit + 2),
+        n_rows=1,
+        n_cols=limit + 2,
+        block_nnz=256,
+        max_segments=1,
+        max_row_nnz=1,
+        opt_buckets=[],
+        transpose=False,
+        index_fallback_policy="auto",
+    )
+
+    def fail_launch(_prepared, _x, use_opt=False, opt_buckets=None):
+        raise RuntimeError("forced native int64 failure")
+
+    monkeypatch.setattr(spmv_mod, "_run_spmv_prepared", fail_launch)
+    x = torch.empty(0, dtype=torch.float32, device=device)
+    with pytest.raises(RuntimeError, match="int32 fallback is unsafe"):
+        spmv_mod._run_spmv_prepared_with_fallback(prepared, x, use_opt=False)
