@@ -71,6 +71,8 @@ def test_only_requested_gradients(device):
     B = torch.randn(300, 16, device=device, requires_grad=True)
     fs.spmm_csr_autograd(data, B, mat).sum().backward()
     assert B.grad is not None
+    assert mat.handle._csc is not None  # dB ran over the CSC permutation
+    mat2 = fs.CsrMatrix(idx, ptr, (300, 300))
     vals = data.clone().requires_grad_(True)
-    fs.spmm_csr_autograd(vals, torch.randn(300, 16, device=device), mat).sum().backward()
-    assert vals.grad is not None and mat.handle._csc is None  # no CSC needed without dB
+    fs.spmm_csr_autograd(vals, torch.randn(300, 16, device=device), mat2).sum().backward()
+    assert vals.grad is not None and mat2.handle._csc is None  # no CSC needed without dB

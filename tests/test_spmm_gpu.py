@@ -168,7 +168,7 @@ def test_spmm_op_validation_errors(device):
         fs.flagsparse_spmm_csr(data, idx, ptr, B_non.double(), (8, 12))
     with pytest.raises(ValueError, match="indptr must be nondecreasing"):
         bad = ptr.clone()
-        bad[3], bad[4] = bad[4] + 1, bad[3]
+        bad[3] = ptr[4] + 1
         fs.flagsparse_spmm_csr(data, idx, bad, B_non, (8, 12))
     with pytest.raises(ValueError, match="max_segments override"):
         fs.flagsparse_spmm_csr(data, idx, ptr, B_non, (8, 12), block_nnz=1, max_segments=1)
