@@ -81,16 +81,41 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void sddmm_bins_kernel(
           }
         }
       }
-#pragma unroll
-      for (int u = 0; u < UNROLL; ++u) dot[u] = group_sum(dot[u], G);
-      if (gl == 0) {
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
-          const int64_t q = p + u * S;
+      if (UNROLL == 4 && G >= 4) {
+        // Transposed reduction of the 4 dot products over the G lanes: two
+        // exchange steps halve the vector each time, then one value per lane
+        // is summed over the aligned sub-group of G/4 lanes.  The total of
+        // dot u ends in lane (u >> 1) * G/2 + (u & 1) * G/4 of the group.
+        const int h = G >> 1, qg = G >> 2;
+        const bool hi = (gl & h) != 0;
+        const A k0 = hi ? dot[2] : dot[0], k1 = hi ? dot[3] : dot[1];
+        const A s0 = hi ? dot[0] : dot[2], s1 = hi ? dot[1] : dot[3];
+        const A e0 = k0 + shfl_xor(s0, h);
+        const A e1 = k1 + shfl_xor(s1, h);
+        const bool hi2 = (gl & qg) != 0;
+        A f = (hi2 ? e1 : e0) + shfl_xor(hi2 ? e0 : e1, qg);
+        f = group_sum(f, qg);
+        if ((gl & (qg - 1)) == 0) {
+          const int u = (hi ? 2 : 0) + (hi2 ? 1 : 0);
+          const int64_t q = p + static_cast<int64_t>(u) * S;
           if (q < w.end) {
-            A r = alpha * dot[u];
+            A r = alpha * f;
             if (data_in != nullptr) r += beta * static_cast<A>(data_in[q]);
             out[q] = static_cast<T>(r);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) dot[u] = group_sum(dot[u], G);
+        if (gl == 0) {
+#pragma unroll
+          for (int u = 0; u < UNROLL; ++u) {
+            const int64_t q = p + u * S;
+            if (q < w.end) {
+              A r = alpha * dot[u];
+              if (data_in != nullptr) r += beta * static_cast<A>(data_in[q]);
+              out[q] = static_cast<T>(r);
+            }
           }
         }
       }

@@ -99,8 +99,10 @@ def test_plan_configs():
     cfg = _plan.spmv_config()
     assert cfg.lanes == (4, 8, 16, 32, 64)
     assert cfg.uppers == (16, 32, 64, 128, _plan.SPMV_LONG_THRESHOLD)
-    cfg = _plan.spmm_config(16)
+    cfg = _plan.spmm_config(16, slot_scale=1)
     assert cfg.lanes == (1, 2, 4) and cfg.uppers == (4, 8, _plan.SPMM_LONG_THRESHOLD)
+    cfg = _plan.spmm_config(16, slot_scale=4)
+    assert cfg.lanes == (1, 2, 4) and cfg.uppers == (16, 32, _plan.SPMM_LONG_THRESHOLD)
     cfg = _plan.spmm_config(64)
     assert cfg.lanes == (1,) and cfg.uppers == (_plan.SPMM_LONG_THRESHOLD,)
     cfg = _plan.spmm_config(1, long_threshold=16)
