@@ -37,8 +37,8 @@ SPMM_UNROLL = 4
 SPMM_LONG_THRESHOLD = 256
 SPMM_SEG_LEN = 256
 SDDMM_UNROLL = 4
-SDDMM_LONG_THRESHOLD = 256
-SDDMM_SEG_LEN = 256
+SDDMM_LONG_THRESHOLD = 1024
+SDDMM_SEG_LEN = 1024
 # Rows per wave over slots per row: measured best on MI355X for power-law rows
 # (mean 14-24 nonzeros) in f32 and bf16.
 SLOT_SCALE = 4
