@@ -6,6 +6,10 @@ path (src/flagsparse/__init__.py:19-154, conf/operators.yaml:46-61, 132-148,
 263-278), with the same arguments and behaviour.  Use it as
 ``import flagsparse_amd as flagsparse``.  The compute is hand-written HIP for
 gfx950 in libflagsparse_hip.so; there is no other backend.
+
+Beyond the reference: the differentiable SpMM (``CsrMatrix``,
+``spmm_csr_autograd``) and the BSR SpMM on the matrix cores
+(``csr_to_bsr``, ``quantize_bsr_fp8``, ``flagsparse_spmm_bsr``).
 """
 
 from ._common import (
@@ -15,6 +19,7 @@ from ._common import (
     validation_metrics,
 )
 from .autograd import CsrMatrix, flagsparse_spmm_csr_autograd, spmm_csr_autograd
+from .bsr import csr_to_bsr, flagsparse_spmm_bsr, quantize_bsr_fp8
 from .sddmm_csr import SDDMMPrepared, flagsparse_sddmm_csr, prepare_sddmm_csr
 from .spmm_csr import (
     SPMM_CSR_ALGORITHMS,
@@ -68,4 +73,7 @@ __all__ = [
     "CsrMatrix",
     "flagsparse_spmm_csr_autograd",
     "spmm_csr_autograd",
+    "csr_to_bsr",
+    "quantize_bsr_fp8",
+    "flagsparse_spmm_bsr",
 ]

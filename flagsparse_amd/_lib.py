@@ -16,7 +16,7 @@ from ._build import LIB_PATH
 FSP_F32, FSP_F64, FSP_F16, FSP_BF16, FSP_F8E4M3 = 0, 1, 2, 3, 4
 FSP_I32, FSP_I64 = 0, 1
 FSP_MAX_BINS = 8
-ABI_VERSION = 1
+ABI_VERSION = 3
 
 _VALUE_CODES = {
     torch.float32: FSP_F32,
@@ -55,6 +55,7 @@ class FspCsrPlan(ctypes.Structure):
         ("long_seg_ptr_dev", _c_vp),
         ("partials_dev", _c_vp),
         ("partials_bytes", _c_i64),
+        ("max_blocks", _c_i64),
     ]
 
 
@@ -83,13 +84,25 @@ SIGNATURES = {
     "fsp_spmm_csr": (
         _c_i32,
         [_c_i32, _c_i32, _c_i32, _c_vp, _c_vp, _c_vp, _c_i32, _c_vp, _c_vp, _c_i64, _c_vp, _c_i64,
-         _c_i64, _c_i64, _c_i32, ctypes.POINTER(FspCsrPlan), _c_vp],
+         _c_i64, _c_i64, _c_i64, _c_i64, _c_i32, ctypes.POINTER(FspCsrPlan), _c_vp],
     ),
     "fsp_sddmm_csr": (
         _c_i32,
         [_c_i32, _c_i32, _c_i32, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp, _c_i64, _c_vp, _c_vp, _c_i64,
-         _c_i64, _c_dbl, _c_dbl, _c_i32, ctypes.POINTER(FspCsrPlan), _c_vp],
+         _c_i64, _c_i64, _c_i64, _c_dbl, _c_dbl, _c_i32, ctypes.POINTER(FspCsrPlan), _c_vp],
     ),
+    "fsp_csr_to_bsr_workspace": (_c_i32, [_c_i64, _c_i64, _c_i64, ctypes.POINTER(_c_i64)]),
+    "fsp_csr_to_bsr_count": (_c_i32, [_c_i32, _c_i32, _c_vp, _c_vp, _c_i64, _c_i64, _c_i64, _c_vp, _c_i64, _c_vp,
+                                      _c_vp]),
+    "fsp_csr_to_bsr_fill": (_c_i32, [_c_i32, _c_i32, _c_i32, _c_vp, _c_vp, _c_vp, _c_i64, _c_i64, _c_i64, _c_i64,
+                                     _c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_i64, _c_vp]),
+    "fsp_absmax": (_c_i32, [_c_i32, _c_vp, _c_i64, _c_i64, _c_i64, _c_vp, _c_vp]),
+    "fsp_bsr_pack_b": (_c_i32, [_c_i32, _c_i32, _c_vp, _c_i64, _c_i64, _c_i64, _c_vp, _c_vp, _c_vp]),
+    "fsp_bsr_quantize_fp8": (_c_i32, [_c_i32, _c_vp, _c_i64, _c_vp, _c_vp, _c_vp]),
+    "fsp_bsr_spmm": (_c_i32, [_c_i32, _c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_i64, _c_i64, _c_vp, _c_vp, _c_i64,
+                              _c_i64, _c_vp, _c_i64, _c_i64, _c_vp]),
+    "fsp_batch_pack": (_c_i32, [_c_i32, _c_i32, _c_i64, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_i64, _c_i64, _c_i32,
+                                _c_vp, _c_i32, _c_vp, _c_vp, _c_vp]),
 }
 
 _lock = threading.Lock()

@@ -54,8 +54,9 @@ class CsrHandle:
         return p
 
     # --------------------------------------------------------------- kernels
-    def spmm(self, data, B, transpose=False, out=None, lanes_n=None, config=None):
-        """C = A B (transpose=False) or A^T B (transpose=True); B row-major."""
+    def spmm(self, data, B, transpose=False, out=None, lanes_n=None, config=None, max_blocks=0):
+        """C = A B (transpose=False) or A^T B (transpose=True); B row-major.
+        max_blocks > 0: persistent launch of at most that many workgroups."""
         n_out = self.n_cols if transpose else self.n_rows
         n_dense = int(B.shape[1])
         device = B.device
@@ -75,7 +76,7 @@ class CsrHandle:
             plan = self.plan(cfg)
             ptr_t, idx_t, perm_t = self.indptr, self.indices, None
         partials = plan.partials(n_dense, _acc_dtype(data.dtype), device)
-        st = plan.struct(partials)
+        st = plan.struct(partials, max_blocks)
         _lib.check(
             _lib.load().fsp_spmm_csr(
                 _lib.value_code(data.dtype),
@@ -92,6 +93,8 @@ class CsrHandle:
                 int(C.stride(0)),
                 n_out,
                 n_dense,
+                self.nnz,
+                int(B.shape[0]),
                 G,
                 st,
                 _lib.stream_ptr(device),
@@ -100,7 +103,7 @@ class CsrHandle:
         )
         return C
 
-    def spmv(self, data, x, transpose=False, out=None, config=None):
+    def spmv(self, data, x, transpose=False, out=None, config=None, max_blocks=0):
         n_out = self.n_cols if transpose else self.n_rows
         y = out if out is not None else torch.empty(n_out, dtype=data.dtype, device=x.device)
         if n_out == 0:
@@ -115,7 +118,7 @@ class CsrHandle:
             ptr_t, idx_t, perm_t = self.indptr, self.indices, None
         acc = torch.float64 if data.dtype in (torch.float32, torch.float64) else torch.float32
         partials = plan.partials(1, acc, x.device)
-        st = plan.struct(partials)
+        st = plan.struct(partials, max_blocks)
         _lib.check(
             _lib.load().fsp_spmv_csr(
                 _lib.value_code(data.dtype),
@@ -136,7 +139,7 @@ class CsrHandle:
         )
         return y
 
-    def sddmm(self, X, Y, alpha=1.0, beta=0.0, data_in=None, out=None, lanes_k=None, config=None):
+    def sddmm(self, X, Y, alpha=1.0, beta=0.0, data_in=None, out=None, lanes_k=None, config=None, max_blocks=0):
         """out[p] = alpha <X[row p], Y[col p]> + beta data_in[p] on this pattern."""
         k = int(X.shape[1])
         device = X.device
@@ -152,7 +155,7 @@ class CsrHandle:
         G = int(lanes_k) if lanes_k else _plan.lanes_for_dense(k, X.element_size())
         cfg = config or _plan.sddmm_config(G)
         plan = self.plan(cfg)
-        st = plan.struct(None)
+        st = plan.struct(None, max_blocks)
         _lib.check(
             _lib.load().fsp_sddmm_csr(
                 _lib.value_code(X.dtype),
@@ -168,6 +171,8 @@ class CsrHandle:
                 out.data_ptr(),
                 self.n_rows,
                 k,
+                self.nnz,
+                int(Y.shape[0]),
                 float(alpha),
                 float(beta),
                 G,

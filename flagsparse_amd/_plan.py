@@ -145,10 +145,13 @@ class CsrPlan:
             self._partials[key] = buf
         return buf
 
-    def struct(self, partials=None):
+    def struct(self, partials=None, max_blocks=0):
+        """The C plan.  max_blocks > 0 makes the launch persistent: at most
+        that many workgroups, each walking a contiguous run of row tiles."""
         st = _lib.FspCsrPlan()
         st.n_bins = len(self.bin_counts)
         st.unroll = self.config.unroll
+        st.max_blocks = int(max_blocks)
         off = 0
         for i, (lanes, count) in enumerate(zip(self.config.lanes, self.bin_counts)):
             st.bins[i].lanes = int(lanes)

@@ -1,0 +1,214 @@
+"""Autotuner for the row-bin CSR kernels, cached per shape class.
+
+The tuned parameters are the ones the kernels take from their plan:
+
+* ``unroll``         nonzeros per lane (SpMV) or per slot (SpMM) per trip --
+                     the nonzeros a wave has in flight (block-nnz);
+* ``slot_scale``     SpMM row-bin thresholds: a row of length n gets the fewest
+                     slots S with n <= S * unroll * slot_scale;
+* ``lanes_n``        SpMM lanes along the dense dimension (block-n =
+                     lanes_n * 16 bytes of B per row and pass);
+* ``long_threshold`` rows longer than this are cut into segments of
+                     ``seg_len`` nonzeros (= long_threshold);
+* ``blocks_per_cu``  0 = one workgroup per row tile, else a persistent grid of
+                     blocks_per_cu workgroups per compute unit, each walking a
+                     contiguous run of rows.
+
+A shape class is (kernel, value / index dtypes, dense width, rounded log2 of
+the mean and maximum row length and of the row count, compute units).  The
+winner of a class is kept in memory and, unless disabled, in a JSON file
+(``FLAGSPARSE_AMD_TUNE_CACHE``, default ``~/.cache/flagsparse_amd/tune.json``)
+whose entries carry the library's ABI version and a hash of the built
+library, so a rebuilt kernel invalidates them.  Candidates are timed with
+device events on the current stream (median of a few launches after one
+warm-up launch).
+"""
+
+import hashlib
+import json
+import math
+import os
+import threading
+from dataclasses import asdict, dataclass
+
+import torch
+
+from . import _lib, _plan
+
+_CACHE_LOCK = threading.Lock()
+_MEMORY = {}
+_LIB_HASH = None
+
+
+@dataclass(frozen=True)
+class TuneParams:
+    unroll: int = 4
+    slot_scale: int = _plan.SLOT_SCALE
+    lanes_n: int = 0  # 0 = lanes_for_dense(n_dense)
+    long_threshold: int = 0  # 0 = the kernel family's default
+    blocks_per_cu: int = 0
+
+    def spmv_config(self):
+        lt = self.long_threshold or _plan.SPMV_LONG_THRESHOLD
+        return _plan.spmv_config(unroll=self.unroll, long_threshold=lt, seg_len=lt)
+
+    def spmm_config(self, lanes_n):
+        lt = self.long_threshold or _plan.SPMM_LONG_THRESHOLD
+        return _plan.spmm_config(lanes_n, unroll=self.unroll, long_threshold=lt, seg_len=lt,
+                                 slot_scale=self.slot_scale)
+
+    def max_blocks(self, device):
+        if self.blocks_per_cu <= 0:
+            return 0
+        return int(self.blocks_per_cu) * compute_units(device)
+
+
+def compute_units(device):
+    return int(torch.cuda.get_device_properties(device).multi_processor_count)
+
+
+def _log2_bucket(v):
+    return int(round(math.log2(v))) if v > 0 else -1
+
+
+def shape_class(kind, value_dtype, index_dtype, indptr_dtype, n_dense, n_rows, nnz, max_row, cus):
+    """Key of a shape class (a string, stable across processes)."""
+    mean = nnz / n_rows if n_rows else 0.0
+    return "|".join([
+        kind,
+        str(value_dtype).replace("torch.", ""),
+        str(index_dtype).replace("torch.", ""),
+        str(indptr_dtype).replace("torch.", ""),
+        f"n{int(n_dense)}",
+        f"mean{_log2_bucket(mean)}",
+        f"max{_log2_bucket(max_row)}",
+        f"rows{_log2_bucket(n_rows)}",
+        f"cu{int(cus)}",
+    ])
+
+
+def candidates(kind, n_dense=1, elem_bytes=4):
+    """Candidate parameter sets of a kernel family (small, fixed grids)."""
+    out = []
+    if kind == "spmv":
+        for unroll in (2, 4, 8):
+            for bpc in (0, 4, 8):
+                out.append(TuneParams(unroll=unroll, blocks_per_cu=bpc))
+        return out
+    g = _plan.lanes_for_dense(n_dense, elem_bytes)
+    lanes = [g] + ([g // 2] if g >= 8 else [])
+    for ln in lanes:
+        for unroll in (4, 8):
+            for scale in (1, 2, 4):
+                for bpc in (0, 8):
+                    out.append(TuneParams(unroll=unroll, slot_scale=scale, lanes_n=ln, blocks_per_cu=bpc))
+    return out
+
+
+# ------------------------------------------------------------------- cache
+def cache_path():
+    p = os.environ.get("FLAGSPARSE_AMD_TUNE_CACHE")
+    if p is not None:
+        return p or None  # "" disables the file
+    return os.path.join(os.path.expanduser("~"), ".cache", "flagsparse_amd", "tune.json")
+
+
+def library_hash():
+    """sha1 of the built library file (entries of another build are stale)."""
+    global _LIB_HASH
+    if _LIB_HASH is None:
+        lib = _lib.load()
+        h = hashlib.sha1()
+        with open(lib._name, "rb") as f:
+            for chunk in iter(lambda: f.read(1 << 20), b""):
+                h.update(chunk)
+        _LIB_HASH = h.hexdigest()[:16]
+    return _LIB_HASH
+
+
+def _version_tag():
+    return f"abi{_lib.ABI_VERSION}-{library_hash()}"
+
+
+def _read_file(path):
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return {}
+    return data if isinstance(data, dict) else {}
+
+
+def lookup(key):
+    with _CACHE_LOCK:
+        hit = _MEMORY.get(key)
+        if hit is not None:
+            return hit
+        path = cache_path()
+        if path:
+            entry = _read_file(path).get(key)
+            if isinstance(entry, dict) and entry.get("version") == _version_tag():
+                try:
+                    params = TuneParams(**entry["params"])
+                except (TypeError, KeyError):
+                    return None
+                _MEMORY[key] = params
+                return params
+    return None
+
+
+def store(key, params, timing_ms=None):
+    with _CACHE_LOCK:
+        _MEMORY[key] = params
+        path = cache_path()
+        if not path:
+            return
+        data = _read_file(path)
+        data[key] = {"version": _version_tag(), "params": asdict(params), "ms": timing_ms}
+        try:
+            os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+            tmp = f"{path}.{os.getpid()}.tmp"
+            with open(tmp, "w") as f:
+                json.dump(data, f, indent=1, sort_keys=True)
+            os.replace(tmp, path)
+        except OSError:
+            pass  # an unwritable cache only costs a re-tune
+
+
+def clear_memory():
+    with _CACHE_LOCK:
+        _MEMORY.clear()
+
+
+# ------------------------------------------------------------------ tuning
+def time_launch(fn, reps=5):
+    """Median device time (ms) of fn() on the current stream after a warm-up."""
+    fn()
+    times = []
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        times.append(a.elapsed_time(b))
+    times.sort()
+    return times[len(times) // 2]
+
+
+def tune(key, cands, run, reps=5):
+    """Time run(params) for every candidate; cache and return the fastest.
+    `run` must launch only (no host syncs) so the events time the kernels."""
+    hit = lookup(key)
+    if hit is not None:
+        return hit, None
+    best, best_ms = None, float("inf")
+    results = []
+    for p in cands:
+        ms = time_launch(lambda: run(p), reps)
+        results.append((p, ms))
+        if ms < best_ms:
+            best, best_ms = p, ms
+    store(key, best, best_ms)
+    return best, results

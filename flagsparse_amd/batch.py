@@ -1,0 +1,263 @@
+"""Batches of CSR matrices: SpMV / SpMM of every matrix in one launch.
+
+No counterpart in the reference, which runs one matrix per call.  A batch is
+a packed offset table over concatenated arrays (``CsrBatch``): matrix m owns
+rows ``row_offsets[m]:row_offsets[m+1]``, columns
+``col_offsets[m]:col_offsets[m+1]`` and nonzeros
+``nnz_offsets[m]:nnz_offsets[m+1]`` of the concatenated ``indices`` /
+``data``; its own row pointer (n_m + 1 local offsets from 0) is
+``indptr[row_offsets[m] + m : row_offsets[m+1] + m + 1]``.  Dense operands and
+results are packed the same way: x / B rows by column offsets, y / C rows by
+row offsets.
+
+At construction the batch is packed once on the GPU into its block-diagonal
+CSR (global row pointer and column ids; the values are shared, so they may
+change between calls) and validated.  ``spmv`` / ``spmm`` are then ONE
+launch of the row-bin kernels over that matrix -- persistent when the
+autotuner picks a capped grid -- and ``graph_spmv`` / ``graph_spmm`` capture
+it in a HIP graph.  Each row is reduced in the same order as in a launch
+over its own matrix with the same parameters, so the batched result is
+bitwise equal to ``spmv_per_matrix`` / ``spmm_per_matrix``, the per-matrix
+launch loop kept as the comparator.
+"""
+
+import numpy as np
+import torch
+
+from . import _lib, _plan, autotune
+from ._matrix import CsrHandle
+
+_VALUE_DTYPES = (torch.float32, torch.float64, torch.float16, torch.bfloat16)
+_ERR_MESSAGES = (
+    (1, "every matrix's indptr must start at 0 and end at its nnz"),
+    (2, "indptr must be nondecreasing"),
+    (4, "indices out of range for their matrix's n_cols"),
+)
+
+
+def _host_offsets(v, name, n=None):
+    a = np.asarray(v.cpu() if torch.is_tensor(v) else v, dtype=np.int64).reshape(-1)
+    if a.size < 2 or (n is not None and a.size != n):
+        raise ValueError(f"{name} must have n_mats + 1 >= 2 entries")
+    if a[0] != 0 or np.any(np.diff(a) < 0):
+        raise ValueError(f"{name} must start at 0 and be nondecreasing")
+    return a
+
+
+class CsrBatch:
+    """A packed batch of CSR matrices (see the module docstring)."""
+
+    def __init__(self, data, indices, indptr, row_offsets, col_offsets=None, nnz_offsets=None):
+        if not all(torch.is_tensor(t) for t in (data, indices, indptr)):
+            raise TypeError("data, indices, indptr must all be torch.Tensor")
+        if data.ndim != 1 or indices.ndim != 1 or indptr.ndim != 1:
+            raise ValueError("data, indices, and indptr must be 1D tensors")
+        if not all(t.is_cuda for t in (data, indices, indptr)):
+            raise ValueError("data, indices, and indptr must be CUDA tensors")
+        if not all(t.device == data.device for t in (indices, indptr)):
+            raise ValueError("data, indices, and indptr must be on the same CUDA device")
+        if data.dtype not in _VALUE_DTYPES:
+            raise TypeError("data dtype must be one of: float16, bfloat16, float32, float64")
+        if indices.dtype not in (torch.int32, torch.int64) or indptr.dtype not in (torch.int32, torch.int64):
+            raise TypeError("indices and indptr dtype must be torch.int32 or torch.int64")
+        if data.numel() != indices.numel():
+            raise ValueError("data and indices must have the same length (nnz)")
+        rows = _host_offsets(row_offsets, "row_offsets")
+        n_mats = rows.size - 1
+        cols = rows if col_offsets is None else _host_offsets(col_offsets, "col_offsets", n_mats + 1)
+        if indptr.numel() != rows[-1] + n_mats:
+            raise ValueError(f"indptr length must be total_rows + n_mats = {rows[-1] + n_mats}, got {indptr.numel()}")
+        if nnz_offsets is None:
+            # each matrix's nnz is the last entry of its own row pointer
+            ends = torch.as_tensor(rows[1:] + np.arange(n_mats), device=indptr.device)
+            nnz = np.concatenate([[0], np.cumsum(indptr[ends].cpu().numpy().astype(np.int64))])
+            nnz = _host_offsets(nnz, "per-matrix nnz (last indptr entries)")
+        else:
+            nnz = _host_offsets(nnz_offsets, "nnz_offsets", n_mats + 1)
+        if nnz[-1] != data.numel():
+            raise ValueError(f"nnz_offsets[-1] must equal nnz={data.numel()}, got {nnz[-1]}")
+        self.device = data.device
+        self.data = data.contiguous()
+        self.local_indices = indices.contiguous()
+        self.local_indptr = indptr.contiguous()
+        self.row_offsets, self.col_offsets, self.nnz_offsets = rows, cols, nnz
+        self.n_mats = int(n_mats)
+        self.total_rows, self.total_cols, self.nnz = int(rows[-1]), int(cols[-1]), int(nnz[-1])
+        self._pack()
+        self._params = {}
+        self._per_matrix = None
+
+    @classmethod
+    def from_matrices(cls, matrices):
+        """Pack a list of (data, indices, indptr, shape) CSR matrices (copies)."""
+        if not matrices:
+            raise ValueError("a batch needs at least one matrix")
+        shapes = [(int(s[0]), int(s[1])) for _, _, _, s in matrices]
+        rows = np.concatenate([[0], np.cumsum([s[0] for s in shapes])])
+        cols = np.concatenate([[0], np.cumsum([s[1] for s in shapes])])
+        nnz = np.concatenate([[0], np.cumsum([int(m[0].numel()) for m in matrices])])
+        data = torch.cat([m[0] for m in matrices])
+        idx_dtype = torch.int64 if any(m[1].dtype == torch.int64 for m in matrices) else torch.int32
+        ptr_dtype = torch.int64 if any(m[2].dtype == torch.int64 for m in matrices) else torch.int32
+        indices = torch.cat([m[1].to(idx_dtype) for m in matrices])
+        indptr = torch.cat([m[2].to(ptr_dtype) for m in matrices])
+        return cls(data, indices, indptr, rows, cols, nnz)
+
+    # ------------------------------------------------------------- packing
+    def _pack(self):
+        dev = self.device
+        ptr_t = torch.int32 if self.nnz <= np.iinfo(np.int32).max else torch.int64
+        idx_t = torch.int32 if self.total_cols <= np.iinfo(np.int32).max else torch.int64
+        self.indptr = torch.empty(self.total_rows + 1, dtype=ptr_t, device=dev)
+        self.indices = torch.empty(self.nnz, dtype=idx_t, device=dev)
+        offs = torch.from_numpy(np.stack([self.row_offsets, self.col_offsets, self.nnz_offsets])).to(dev)
+        err = torch.empty(1, dtype=torch.int32, device=dev)
+        _lib.check(
+            _lib.load().fsp_batch_pack(
+                _lib.index_code(self.local_indptr.dtype), _lib.index_code(self.local_indices.dtype), self.n_mats,
+                offs[0].data_ptr(), offs[1].data_ptr(), offs[2].data_ptr(), self.local_indptr.data_ptr(),
+                _lib.ptr(self.local_indices), self.total_rows, self.nnz, _lib.index_code(ptr_t),
+                self.indptr.data_ptr(), _lib.index_code(idx_t), _lib.ptr(self.indices), err.data_ptr(),
+                _lib.stream_ptr(dev),
+            ),
+            "fsp_batch_pack",
+        )
+        bits = int(err.item())  # host sync, construction time only
+        for bit, msg in _ERR_MESSAGES:
+            if bits & bit:
+                if bit == 4:
+                    raise IndexError(msg)
+                raise ValueError(msg)
+        self.handle = CsrHandle(self.indices, self.indptr, (self.total_rows, self.total_cols))
+        lengths = self.indptr[1:] - self.indptr[:-1] if self.total_rows else self.indptr[:0]
+        self.max_row = int(lengths.max().item()) if self.total_rows else 0
+
+    # ------------------------------------------------------------- tuning
+    def params(self, kind, n_dense=1, dtype=None, tune=True):
+        """Kernel parameters of this batch for `kind` ("spmv" / "spmm"): the
+        cached winner of its shape class, else tuned now (tune=True) or the
+        defaults."""
+        dtype = dtype or self.data.dtype
+        cus = autotune.compute_units(self.device)
+        key = autotune.shape_class(kind, dtype, self.indices.dtype, self.indptr.dtype, n_dense, self.total_rows,
+                                   self.nnz, self.max_row, cus)
+        p = self._params.get(key) or autotune.lookup(key)
+        if p is None:
+            if not tune:
+                return autotune.TuneParams()
+            p = self._tune(kind, key, n_dense, dtype)
+        self._params[key] = p
+        return p
+
+    def _tune(self, kind, key, n_dense, dtype):
+        elem = torch.empty(0, dtype=dtype).element_size()
+        cands = autotune.candidates(kind, n_dense, elem)
+        data = self.data if self.data.dtype == dtype else self.data.to(dtype)
+        if kind == "spmv":
+            x = torch.ones(self.total_cols, dtype=dtype, device=self.device)
+            y = torch.empty(self.total_rows, dtype=dtype, device=self.device)
+            for p in cands:  # plans are built (host syncs) before any timing
+                self.handle.plan(p.spmv_config())
+
+            def run(p):
+                self.handle.spmv(data, x, out=y, config=p.spmv_config(), max_blocks=p.max_blocks(self.device))
+        else:
+            B = torch.ones((self.total_cols, n_dense), dtype=dtype, device=self.device)
+            C = torch.empty((self.total_rows, n_dense), dtype=dtype, device=self.device)
+            for p in cands:
+                self.handle.plan(p.spmm_config(self._lanes(p, n_dense, elem)))
+
+            def run(p):
+                self.handle.spmm(data, B, out=C, lanes_n=self._lanes(p, n_dense, elem),
+                                 config=p.spmm_config(self._lanes(p, n_dense, elem)),
+                                 max_blocks=p.max_blocks(self.device))
+        best, self.tune_results = autotune.tune(key, cands, run)
+        return best
+
+    @staticmethod
+    def _lanes(p, n_dense, elem):
+        return p.lanes_n or _plan.lanes_for_dense(n_dense, elem)
+
+    # ------------------------------------------------------------ kernels
+    def _check_dense(self, t, n, what):
+        if not torch.is_tensor(t) or not t.is_cuda or t.device != self.device:
+            raise ValueError(f"{what} must be a CUDA tensor on the batch's device")
+        if t.shape[0] != n:
+            raise ValueError(f"{what}.shape[0] must be {n}, got {t.shape[0]}")
+
+    def spmv(self, x, out=None, data=None, params=None):
+        """y (packed by rows) = blockdiag(A_m) x (packed by columns): one launch."""
+        data = self.data if data is None else data
+        self._check_dense(x, self.total_cols, "x")
+        if x.ndim != 1 or x.dtype != data.dtype:
+            raise ValueError("x must be 1D with the values' dtype")
+        p = params or self.params("spmv", 1, data.dtype)
+        return self.handle.spmv(data, x.contiguous(), out=out, config=p.spmv_config(),
+                                max_blocks=p.max_blocks(self.device))
+
+    def spmm(self, B, out=None, data=None, params=None):
+        """C (packed by rows) = blockdiag(A_m) B (packed by rows): one launch."""
+        data = self.data if data is None else data
+        self._check_dense(B, self.total_cols, "B")
+        if B.ndim != 2 or B.dtype != data.dtype:
+            raise ValueError("B must be 2D with the values' dtype")
+        n_dense = int(B.shape[1])
+        p = params or self.params("spmm", n_dense, data.dtype)
+        lanes = self._lanes(p, n_dense, data.element_size())
+        return self.handle.spmm(data, B.contiguous(), out=out, lanes_n=lanes, config=p.spmm_config(lanes),
+                                max_blocks=p.max_blocks(self.device))
+
+    # ----------------------------------------------------- graph capture
+    def _capture(self, fn):
+        fn()  # plans, partials and parameters exist before capture
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            fn()
+        return g
+
+    def graph_spmv(self, x, y, data=None):
+        """HIP graph of spmv(x, out=y); replay() reruns it on the current x."""
+        return self._capture(lambda: self.spmv(x, out=y, data=data))
+
+    def graph_spmm(self, B, C, data=None):
+        return self._capture(lambda: self.spmm(B, out=C, data=data))
+
+    # ------------------------------------------- per-matrix comparator
+    def _matrices(self):
+        if self._per_matrix is None:
+            mats = []
+            for m in range(self.n_mats):
+                r0, r1 = int(self.row_offsets[m]), int(self.row_offsets[m + 1])
+                n0, n1 = int(self.nnz_offsets[m]), int(self.nnz_offsets[m + 1])
+                c0, c1 = int(self.col_offsets[m]), int(self.col_offsets[m + 1])
+                h = CsrHandle(self.local_indices[n0:n1], self.local_indptr[r0 + m: r1 + m + 1], (r1 - r0, c1 - c0))
+                mats.append((h, r0, r1, n0, n1, c0, c1))
+            self._per_matrix = mats
+        return self._per_matrix
+
+    def spmv_per_matrix(self, x, out=None, data=None, params=None):
+        """The same product as spmv, one launch per matrix (the comparator)."""
+        data = self.data if data is None else data
+        p = params or self.params("spmv", 1, data.dtype)
+        y = out if out is not None else torch.empty(self.total_rows, dtype=data.dtype, device=self.device)
+        cfg = p.spmv_config()
+        for h, r0, r1, n0, n1, c0, c1 in self._matrices():
+            h.spmv(data[n0:n1], x[c0:c1], out=y[r0:r1], config=cfg)
+        return y
+
+    def spmm_per_matrix(self, B, out=None, data=None, params=None):
+        data = self.data if data is None else data
+        n_dense = int(B.shape[1])
+        p = params or self.params("spmm", n_dense, data.dtype)
+        lanes = self._lanes(p, n_dense, data.element_size())
+        cfg = p.spmm_config(lanes)
+        C = out if out is not None else torch.empty((self.total_rows, n_dense), dtype=data.dtype,
+                                                    device=self.device)
+        for h, r0, r1, n0, n1, c0, c1 in self._matrices():
+            h.spmm(data[n0:n1], B[c0:c1], out=C[r0:r1], lanes_n=lanes, config=cfg)
+        return C
+
+    def split(self, packed):
+        """Per-matrix views of a packed result (rows by row offsets)."""
+        return [packed[int(self.row_offsets[m]): int(self.row_offsets[m + 1])] for m in range(self.n_mats)]

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #define FSP_WG_THREADS 256
 #define FSP_WAVES (FSP_WG_THREADS / 64)
 
@@ -115,6 +117,30 @@ template <typename A> __device__ __forceinline__ A group_sum(A v, int lanes) {
   if (lanes >= 32) v += shfl_xor(v, 16);
   if (lanes >= 64) v += shfl_xor(v, 32);
   return v;
+}
+
+// Byte-offset addressing: 32-bit offsets from a uniform base when OFF32.
+template <bool OFF32> struct addr;
+// (char-pointer arithmetic keeps the global address space visible to the
+// compiler, so the loads stay global_load with a scalar base.)
+template <> struct addr<true> {
+  using off_t = uint32_t;
+  template <typename P> static __device__ __forceinline__ P* at(P* base, off_t byte_off) {
+    using C = typename std::conditional<std::is_const<P>::value, const char, char>::type;
+    return reinterpret_cast<P*>(reinterpret_cast<C*>(base) + static_cast<uint64_t>(byte_off));
+  }
+};
+template <> struct addr<false> {
+  using off_t = uint64_t;
+  template <typename P> static __device__ __forceinline__ P* at(P* base, off_t byte_off) {
+    using C = typename std::conditional<std::is_const<P>::value, const char, char>::type;
+    return reinterpret_cast<P*>(reinterpret_cast<C*>(base) + byte_off);
+  }
+};
+
+template <bool OFF32, typename P>
+__device__ __forceinline__ const P& elem(const P* base, typename addr<OFF32>::off_t i) {
+  return *addr<OFF32>::at(base, i * static_cast<typename addr<OFF32>::off_t>(sizeof(P)));
 }
 
 // ------------------------------------------------------------ index helpers

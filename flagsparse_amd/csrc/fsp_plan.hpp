@@ -3,8 +3,11 @@
 // One launch covers every row bin of a plan plus the long-row segments: the
 // table (passed by value as a kernel argument) gives each bin its range of
 // workgroups; a workgroup finds its bin with a uniform scan of <= 9 entries.
-// Inside a bin, workgroup j processes row tiles j, j + nblocks, ... so the
-// grid stays bounded for any row count.
+// Inside a bin, workgroup j processes the contiguous row tiles
+// [j * chunk, (j + 1) * chunk).  chunk is 1 unless the plan caps the grid
+// (max_blocks, a persistent launch sized to the chip): then each workgroup
+// walks a contiguous run of rows, which keeps the dense rows a run of the
+// matrix gathers (one matrix of a packed batch, say) in one XCD's L2.
 #pragma once
 
 #include "fsp_device.hpp"
@@ -20,6 +23,7 @@ struct launch_bin {
   int64_t count;         // rows (or segments) in the bin
   int64_t block_begin;   // first workgroup of this bin
   int64_t nblocks;       // workgroups of this bin
+  int64_t chunk;         // consecutive tiles per workgroup
   int32_t lanes_log2;    // SpMV: lanes per row; SpMM/SDDMM: nonzero slots per row
   int32_t is_segment;    // 1 = items are long-row segments
 };
@@ -75,41 +79,54 @@ static inline int ilog2(int64_t v) {
 
 // Builds the launch table.  `items_per_block(lanes_log2, is_segment)` gives
 // the rows (segments) one workgroup covers per tile.  `seg_lanes_log2` is the
-// lanes_log2 used for segments.  Returns the total number of workgroups.
+// lanes_log2 used for segments.  plan->max_blocks > 0 caps the total grid:
+// each bin then gets a share of the workgroups proportional to its tiles
+// (at least one) and every workgroup a contiguous chunk of tiles.  Returns
+// the total number of workgroups.
 template <typename F>
 static int64_t build_launch_table(const fsp_csr_plan* plan, int seg_lanes_log2, int max_lanes_log2,
                                   F&& items_per_block, launch_table* t) {
-  int64_t blocks = 0;
+  int64_t tiles[kMaxLaunchBins];
+  int64_t total_tiles = 0;
   t->n = 0;
   for (int i = 0; i < plan->n_bins && i < FSP_MAX_BINS; ++i) {
     const fsp_bin& b = plan->bins[i];
     if (b.count <= 0) continue;
-    launch_bin& lb = t->b[t->n++];
+    launch_bin& lb = t->b[t->n];
     int ll = ilog2(b.lanes > 0 ? b.lanes : 1);
     if (ll > max_lanes_log2) ll = max_lanes_log2;
     lb.rows = b.rows;
     lb.count = b.count;
     lb.lanes_log2 = ll;
     lb.is_segment = 0;
-    const int64_t per = items_per_block(ll, 0);
-    int64_t nb = ceil_div(b.count, per);
-    if (nb > kMaxBlocksPerBin) nb = kMaxBlocksPerBin;
-    lb.block_begin = blocks;
-    lb.nblocks = nb;
-    blocks += nb;
+    tiles[t->n] = ceil_div(b.count, items_per_block(ll, 0));
+    total_tiles += tiles[t->n];
+    ++t->n;
   }
   if (plan->n_segments > 0) {
-    launch_bin& lb = t->b[t->n++];
+    launch_bin& lb = t->b[t->n];
     lb.rows = nullptr;
     lb.count = plan->n_segments;
     lb.lanes_log2 = seg_lanes_log2;
     lb.is_segment = 1;
-    const int64_t per = items_per_block(seg_lanes_log2, 1);
-    int64_t nb = ceil_div(plan->n_segments, per);
+    tiles[t->n] = ceil_div(plan->n_segments, items_per_block(seg_lanes_log2, 1));
+    total_tiles += tiles[t->n];
+    ++t->n;
+  }
+  const int64_t cap = plan->max_blocks;
+  int64_t blocks = 0;
+  for (int i = 0; i < t->n; ++i) {
+    launch_bin& lb = t->b[i];
+    int64_t nb = tiles[i];
+    if (cap > 0 && total_tiles > cap) {
+      nb = tiles[i] * cap / total_tiles;  // tiles <= 2^40, cap <= 2^20: no overflow
+      if (nb < 1) nb = 1;
+    }
     if (nb > kMaxBlocksPerBin) nb = kMaxBlocksPerBin;
+    lb.chunk = ceil_div(tiles[i], nb);
+    lb.nblocks = ceil_div(tiles[i], lb.chunk);  // no workgroup without tiles
     lb.block_begin = blocks;
-    lb.nblocks = nb;
-    blocks += nb;
+    blocks += lb.nblocks;
   }
   t->seg_row = plan->seg_row_dev;
   t->seg_begin = plan->seg_begin_dev;
@@ -134,6 +151,10 @@ static inline int check_plan(const fsp_csr_plan* plan, int64_t n_rows) {
                 (long long)total, (long long)plan->n_long_rows, (long long)n_rows);
   }
   if (plan->n_segments < 0 || plan->n_long_rows < 0) return fail(FSP_ERR_ARG, "plan: negative segment counts");
+  if (plan->max_blocks < 0 || plan->max_blocks > kMaxBlocksPerBin) {
+    return fail(FSP_ERR_ARG, "plan: max_blocks %lld outside [0, %lld]", (long long)plan->max_blocks,
+                (long long)kMaxBlocksPerBin);
+  }
   if (plan->n_segments > 0 && (plan->seg_row_dev == nullptr || plan->seg_begin_dev == nullptr ||
                                plan->seg_end_dev == nullptr)) {
     return fail(FSP_ERR_ARG, "plan: segment arrays missing");

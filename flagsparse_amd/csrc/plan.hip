@@ -164,20 +164,45 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void csc_colptr_kernel(
   }
 }
 
-// rowind[q] = row r with indptr[r] <= perm[q] < indptr[r+1].
-template <typename PT, typename QT, typename RT>
-__global__ __launch_bounds__(FSP_WG_THREADS) void csc_rowind_kernel(
-    const PT* __restrict__ indptr, int64_t n_rows, const QT* __restrict__ perm, int64_t nnz,
-    RT* __restrict__ rowind) {
+// Largest r in [lo, hi) with indptr[r] <= p (indptr[lo] <= p guaranteed).
+template <typename PT>
+__device__ __forceinline__ int64_t row_search(const PT* __restrict__ indptr, int64_t lo, int64_t hi, int64_t p) {
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (static_cast<int64_t>(indptr[mid]) <= p) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// row_of[p] = row r with indptr[r] <= p < indptr[r+1].  A workgroup covers
+// kExpandNnz consecutive nonzeros: it bounds their rows with two searches over
+// the whole indptr, then every thread searches only that (small, cached) range.
+constexpr int kExpandNnz = 1024;
+template <typename PT, typename RT>
+__global__ __launch_bounds__(FSP_WG_THREADS) void csc_row_of_kernel(const PT* __restrict__ indptr, int64_t n_rows,
+                                                                    int64_t nnz, RT* __restrict__ row_of) {
+  __shared__ int64_t s_lo, s_hi;
+  const int64_t b0 = static_cast<int64_t>(blockIdx.x) * kExpandNnz;
+  const int64_t b1 = b0 + kExpandNnz < nnz ? b0 + kExpandNnz : nnz;
+  if (threadIdx.x == 0) {
+    s_lo = row_search(indptr, 0, n_rows, b0);
+    s_hi = row_search(indptr, 0, n_rows, b1 - 1) + 1;
+  }
+  __syncthreads();
+  const int64_t lo = s_lo, hi = s_hi;
+  for (int64_t p = b0 + threadIdx.x; p < b1; p += FSP_WG_THREADS) {
+    row_of[p] = static_cast<RT>(row_search(indptr, lo, hi, p));
+  }
+}
+
+// rowind[q] = row_of[perm[q]].
+template <typename QT, typename RT>
+__global__ __launch_bounds__(FSP_WG_THREADS) void csc_rowind_kernel(const RT* __restrict__ row_of,
+                                                                    const QT* __restrict__ perm, int64_t nnz,
+                                                                    RT* __restrict__ rowind) {
   for (int64_t q = static_cast<int64_t>(blockIdx.x) * FSP_WG_THREADS + threadIdx.x; q < nnz;
        q += static_cast<int64_t>(gridDim.x) * FSP_WG_THREADS) {
-    const int64_t p = static_cast<int64_t>(perm[q]);
-    int64_t lo = 0, hi = n_rows;  // invariant: indptr[lo] <= p < indptr[hi]
-    while (hi - lo > 1) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (static_cast<int64_t>(indptr[mid]) <= p) lo = mid; else hi = mid;
-    }
-    rowind[q] = static_cast<RT>(lo);
+    rowind[q] = row_of[static_cast<int64_t>(perm[q])];
   }
 }
 
@@ -226,7 +251,7 @@ static int csc_sort_bytes(int64_t nnz, int64_t n_cols, size_t* bytes) {
 }
 
 struct csc_layout {
-  int64_t keys_in, keys_out, vals_in, sort_tmp, sort_bytes, total;
+  int64_t keys_in, keys_out, vals_in, row_of, sort_tmp, sort_bytes, total;
 };
 
 template <typename KT, typename QT>
@@ -238,6 +263,7 @@ static int csc_workspace_layout(int64_t nnz, int64_t n_cols, csc_layout* L) {
   L->keys_in = off;  off = align_up(off + nnz * (int64_t)sizeof(KT), 256);
   L->keys_out = off; off = align_up(off + nnz * (int64_t)sizeof(KT), 256);
   L->vals_in = off;  off = align_up(off + nnz * (int64_t)sizeof(QT), 256);
+  L->row_of = off;   off = align_up(off + nnz * (int64_t)sizeof(int64_t), 256);
   L->sort_tmp = off; off = align_up(off + (int64_t)sort_bytes, 256);
   L->sort_bytes = (int64_t)sort_bytes;
   L->total = off + 256;
@@ -404,9 +430,15 @@ int fsp_csr_to_csc(int32_t indptr_type, int32_t index_type, const void* indptr_d
     if (nnz > 0) {
       FSP_DISPATCH_INDEX(indptr_type, PT, {
         FSP_DISPATCH_INDEX(rowind_type, RT, {
-          hipLaunchKernelGGL((csc_rowind_kernel<PT, QT, RT>), dim3(grid_for(nnz)), dim3(FSP_WG_THREADS), 0,
-                             s, static_cast<const PT*>(indptr_dev), n_rows, perm, nnz,
-                             static_cast<RT*>(rowind_dev));
+          RT* row_of = reinterpret_cast<RT*>(base + L.row_of);
+          const int64_t nblk = ceil_div(nnz, kExpandNnz);
+          if (nblk > INT32_MAX) return fail(FSP_ERR_ARG, "fsp_csr_to_csc: nnz too large");
+          hipLaunchKernelGGL((csc_row_of_kernel<PT, RT>), dim3(static_cast<unsigned>(nblk)), dim3(FSP_WG_THREADS),
+                             0, s, static_cast<const PT*>(indptr_dev), n_rows, nnz, row_of);
+          st = check_launch("csc_row_of");
+          if (st) return st;
+          hipLaunchKernelGGL((csc_rowind_kernel<QT, RT>), dim3(grid_for(nnz)), dim3(FSP_WG_THREADS), 0, s, row_of,
+                             perm, nnz, static_cast<RT*>(rowind_dev));
           st = check_launch("csc_rowind");
           if (st) return st;
         });

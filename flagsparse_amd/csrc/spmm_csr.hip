@@ -11,18 +11,27 @@
 // partial rows a second pass adds in segment order (atomic-free).
 // Over a CSC with perm != NULL the same kernel computes A^T B, reading the
 // values as data[perm[q]] (no transposed copy of A).
+//
+// The loop is software-pipelined: while the B-row loads of one trip are in
+// flight a lane already loads the column ids / values of its next trip -- of
+// the same row, or the first trip of its next row -- so the index stream's
+// latency hides behind the gathers.  When every operand fits in 4 GiB
+// (OFF32), addresses are 32-bit byte offsets from the (uniform) base
+// pointers: one VGPR per address, which keeps occupancy at 8 waves/SIMD.
 #include "fsp_device.hpp"
 #include "fsp_host.hpp"
 #include "fsp_plan.hpp"
 
 namespace fsp {
 
-template <typename T, typename IT, typename PT, typename QT, int VEC, int UNROLL, bool PERM>
+template <typename T, typename IT, typename PT, typename QT, int VEC, int UNROLL, bool PERM, bool OFF32>
 __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
     const PT* __restrict__ indptr, const IT* __restrict__ indices, const T* __restrict__ data,
     const QT* __restrict__ perm, const T* __restrict__ B, int64_t ldb, T* __restrict__ C, int64_t ldc,
     int64_t n_dense, int g_log2, typename acc_of<T>::type* __restrict__ partials, launch_table tab) {
   using A = typename acc_of<T>::type;
+  using O = typename addr<OFF32>::off_t;  // element / byte offsets
+  using P = typename std::conditional<OFF32, int32_t, int64_t>::type;  // nonzero positions
   const int k = find_bin(tab, blockIdx.x);
   const launch_bin& lb = tab.b[k];
   const int s_log2 = lb.lanes_log2;
@@ -35,92 +44,191 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
   const int sub = lane >> (g_log2 + s_log2);
   const int64_t per_wave = 64 >> (g_log2 + s_log2);
   const int64_t per_block = per_wave * FSP_WAVES;
-  const int64_t col0 = (static_cast<int64_t>(blockIdx.y) * G + gl) * VEC;
-  const bool col_ok = col0 < n_dense;
-  for (int64_t tile = blockIdx.x - lb.block_begin; tile * per_block < lb.count; tile += lb.nblocks) {
-    const int64_t rr = tile * per_block + wave * per_wave + sub;
-    const bool active = rr < lb.count;
+  const O col0 = static_cast<O>((static_cast<int64_t>(blockIdx.y) * G + gl) * VEC);
+  const bool col_ok = static_cast<int64_t>(col0) < n_dense;
+  // this workgroup's contiguous run of tiles: items [i_begin, i_end)
+  const int64_t i_begin = (static_cast<int64_t>(blockIdx.x) - lb.block_begin) * lb.chunk * per_block;
+  const int64_t i_end = i_begin + lb.chunk * per_block < lb.count ? i_begin + lb.chunk * per_block : lb.count;
+  const O ldb_o = static_cast<O>(ldb);
+  const O ldc_o = static_cast<O>(ldc);
+  const O nd_o = static_cast<O>(n_dense);
+
+  // Every load below is unconditional (indices clamped into range, results
+  // masked by selects) so the compiler can count outstanding loads exactly
+  // and wait only for the B rows before the FMAs, not for the prefetch.
+  const O col_ld = col_ok ? col0 : O(0);  // lanes past n_dense read column 0
+  const int64_t last_item = lb.count - 1;  // bins in the table are non-empty
+
+  // Item (row or segment) of this lane: output slot, nonzero range.
+  auto item = [&](int64_t r, O& out, P& beg, P& end) {
+    const bool live = r < lb.count && col_ok;
+    const int64_t rc = r < lb.count ? r : last_item;
+    O o;
+    P b, e;
+    if (lb.is_segment) {
+      o = static_cast<O>(rc);
+      b = static_cast<P>(tab.seg_begin[rc]);
+      e = static_cast<P>(tab.seg_end[rc]);
+    } else {
+      const int64_t row = lb.rows ? static_cast<int64_t>(lb.rows[rc]) : rc;
+      o = static_cast<O>(row);
+      b = static_cast<P>(indptr[row]);
+      e = static_cast<P>(indptr[row + 1]);
+    }
+    out = o;
+    beg = live ? b : P(0);
+    end = live ? e : P(0);
+  };
+  // Column ids and values of nonzeros p, p+S, ... (zero past `end`).
+  auto trip = [&](P p, P end, IT (&c)[UNROLL], A (&a)[UNROLL]) {
+    const P last = end > 0 ? end - 1 : P(0);
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      const P q = p + static_cast<P>(u) * S;
+      const bool ok = q < end;
+      const O qc = static_cast<O>(ok ? q : last);
+      const IT ci = elem<OFF32>(indices, qc);
+      const O qv = PERM ? static_cast<O>(elem<OFF32>(perm, qc)) : qc;
+      const A ai = static_cast<A>(elem<OFF32>(data, qv));
+      c[u] = ok ? ci : IT(0);
+      a[u] = ok ? ai : A(0);
+    }
+  };
+
+  int64_t rr = i_begin + wave * per_wave + sub;
+  O out;
+  P beg, end;
+  item(rr, out, beg, end);
+  IT c[UNROLL];
+  A a[UNROLL];
+  trip(beg + slot, end, c, a);
+
+  for (int64_t base = i_begin; base < i_end; base += per_block) {  // wave-uniform
+    const bool active = rr < lb.count && col_ok;
+    const int64_t rr_next = rr + per_block;
+    O out_n;
+    P beg_n, end_n;
+    item(rr_next, out_n, beg_n, end_n);
     A acc[VEC];
 #pragma unroll
     for (int v = 0; v < VEC; ++v) acc[v] = A(0);
-    work_item w{0, 0, 0, 0};
-    if (active) w = load_item(tab, lb, indptr, rr);
-    if (active && col_ok) {
-      const T* Bc = B + col0;
-      for (int64_t p = w.begin + slot; p < w.end; p += S * UNROLL) {
-        int64_t c[UNROLL];
-        A a[UNROLL];
+    bool prefetched = false;
+    for (P p = beg + slot; p < end; p += static_cast<P>(S) * UNROLL) {
+      vec_t<T, VEC> b[UNROLL];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
-          const int64_t q = p + u * S;
-          const bool ok = q < w.end;
-          const int64_t qc = ok ? q : p;
-          c[u] = static_cast<int64_t>(indices[qc]);
-          const T av = data[PERM ? static_cast<int64_t>(perm[qc]) : qc];
-          a[u] = ok ? static_cast<A>(av) : A(0);
-        }
-        vec_t<T, VEC> b[UNROLL];
+      for (int u = 0; u < UNROLL; ++u) {
+        const O boff = (static_cast<O>(c[u]) * ldb_o + col_ld) * static_cast<O>(sizeof(T));
+        b[u] = *addr<OFF32>::at(reinterpret_cast<const vec_t<T, VEC>*>(B), boff);
+      }
+      // prefetch: the next trip of this item, else the first trip of the next
+      const P pn = p + static_cast<P>(S) * UNROLL;
+      const bool same = pn < end;
+      prefetched = prefetched || !same;
+      A an[UNROLL];
+      IT cn[UNROLL];
+      trip(same ? pn : beg_n + slot, same ? end : end_n, cn, an);
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
-          if (p + u * S < w.end) {
-            b[u] = load_vec<T, VEC>(Bc + c[u] * ldb);
-          } else {
+      for (int u = 0; u < UNROLL; ++u) {
+        const bool ok = p + static_cast<P>(u) * S < end;
 #pragma unroll
-            for (int v = 0; v < VEC; ++v) b[u].v[v] = T(0);
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
-#pragma unroll
-          for (int v = 0; v < VEC; ++v) acc[v] += a[u] * static_cast<A>(b[u].v[v]);
+        for (int v = 0; v < VEC; ++v) {
+          const A bv = ok ? static_cast<A>(b[u].v[v]) : A(0);
+          acc[v] += a[u] * bv;
         }
       }
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        c[u] = cn[u];
+        a[u] = an[u];
+      }
     }
+    if (!prefetched) trip(beg_n + slot, end_n, c, a);
     // Sum the S slots of a row: lanes that differ only in the slot bits.
     for (int j = 0; j < s_log2; ++j) {
 #pragma unroll
       for (int v = 0; v < VEC; ++v) acc[v] += shfl_xor(acc[v], G << j);
     }
-    if (active && col_ok && slot == 0) {
+    if (active && slot == 0) {
       if (lb.is_segment) {
-        A* dst = partials + w.out * n_dense + col0;
+        A* dst = addr<OFF32>::at(partials, (out * nd_o + col0) * static_cast<O>(sizeof(A)));
 #pragma unroll
         for (int v = 0; v < VEC; ++v) dst[v] = acc[v];
       } else {
         vec_t<T, VEC> o;
 #pragma unroll
         for (int v = 0; v < VEC; ++v) o.v[v] = static_cast<T>(acc[v]);
-        store_vec<T, VEC>(C + w.out * ldc + col0, o);
+        *addr<OFF32>::at(reinterpret_cast<vec_t<T, VEC>*>(C), (out * ldc_o + col0) * static_cast<O>(sizeof(T))) = o;
       }
     }
+    out = out_n;
+    beg = beg_n;
+    end = end_n;
+    rr = rr_next;
   }
 }
 
 // C[long_rows[i], :] = sum of the row's segment partials, in segment order.
+// One thread per (long row, column); a workgroup covers 256 consecutive
+// (row, column) pairs of the flattened [n_long, n_dense] index space.
 template <typename T>
 __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_long_reduce_kernel(
     const typename acc_of<T>::type* __restrict__ partials, const int32_t* __restrict__ long_rows,
     const int64_t* __restrict__ seg_ptr, int64_t n_long, int64_t n_dense, T* __restrict__ C, int64_t ldc) {
   using A = typename acc_of<T>::type;
-  for (int64_t i = blockIdx.y; i < n_long; i += gridDim.y) {
+  const int64_t total = n_long * n_dense;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * FSP_WG_THREADS + threadIdx.x; t < total;
+       t += static_cast<int64_t>(gridDim.x) * FSP_WG_THREADS) {
+    const int64_t i = t / n_dense;
+    const int64_t col = t - i * n_dense;
     const int64_t s0 = seg_ptr[i], s1 = seg_ptr[i + 1];
-    const int64_t row = long_rows[i];
-    for (int64_t col = static_cast<int64_t>(blockIdx.x) * FSP_WG_THREADS + threadIdx.x; col < n_dense;
-         col += static_cast<int64_t>(gridDim.x) * FSP_WG_THREADS) {
-      A acc = A(0);
-      for (int64_t sg = s0; sg < s1; ++sg) acc += partials[sg * n_dense + col];
-      C[row * ldc + col] = static_cast<T>(acc);
-    }
+    A acc = A(0);
+    for (int64_t sg = s0; sg < s1; ++sg) acc += partials[sg * n_dense + col];
+    C[static_cast<int64_t>(long_rows[i]) * ldc + col] = static_cast<T>(acc);
   }
 }
 
+// C[r, :n_dense] = 0 for every row (the matrix has no nonzeros).
+template <typename T>
+__global__ __launch_bounds__(FSP_WG_THREADS) void fill_zero_rows_kernel(T* __restrict__ C, int64_t ldc,
+                                                                        int64_t n_rows, int64_t n_dense) {
+  const int64_t total = n_rows * n_dense;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * FSP_WG_THREADS + threadIdx.x; t < total;
+       t += static_cast<int64_t>(gridDim.x) * FSP_WG_THREADS) {
+    const int64_t r = t / n_dense;
+    C[r * ldc + (t - r * n_dense)] = T(0);
+  }
+}
+
+struct spmm_args {
+  const void* indptr;
+  const void* indices;
+  const void* data;
+  const void* perm;
+  const void* B;
+  int64_t ldb;
+  void* C;
+  int64_t ldc;
+  int64_t n_rows;
+  int64_t n_dense;
+  int lanes_n;
+  int64_t nnz;
+  int64_t b_rows;
+};
+
+template <typename T, typename IT, typename PT, typename QT, int VEC, int UNROLL, bool PERM, bool OFF32>
+static void launch_bins(const spmm_args& x, dim3 grid, int g_log2, typename acc_of<T>::type* partials,
+                        const launch_table& tab, hipStream_t s) {
+  hipLaunchKernelGGL((spmm_bins_kernel<T, IT, PT, QT, VEC, UNROLL, PERM, OFF32>), grid, dim3(FSP_WG_THREADS), 0, s,
+                     static_cast<const PT*>(x.indptr), static_cast<const IT*>(x.indices),
+                     static_cast<const T*>(x.data), static_cast<const QT*>(x.perm), static_cast<const T*>(x.B),
+                     x.ldb, static_cast<T*>(x.C), x.ldc, x.n_dense, g_log2, partials, tab);
+}
+
 template <typename T, typename IT, typename PT, typename QT, int VEC, bool PERM>
-static int launch_spmm(const void* indptr, const void* indices, const void* data, const void* perm,
-                       const void* B, int64_t ldb, void* C, int64_t ldc, int64_t n_dense, int lanes_n,
-                       const fsp_csr_plan* plan, hipStream_t s) {
+static int launch_spmm(const spmm_args& x, const fsp_csr_plan* plan, hipStream_t s) {
   using A = typename acc_of<T>::type;
-  const int64_t vcols = ceil_div(n_dense, VEC);
-  int G = lanes_n;
+  const int64_t vcols = ceil_div(x.n_dense, VEC);
+  int G = x.lanes_n;
   if (G <= 0) {
     G = 1;
     while (G < 64 && G < vcols) G <<= 1;
@@ -132,39 +240,41 @@ static int launch_spmm(const void* indptr, const void* indices, const void* data
   const int64_t blocks = build_launch_table(
       plan, max_s_log2, max_s_log2,
       [g_log2](int sl, int) { return int64_t(FSP_WAVES) * (int64_t(64) >> (g_log2 + sl)); }, &tab);
-  if (plan->n_segments > 0 &&
-      (plan->partials_dev == nullptr || plan->partials_bytes < plan->n_segments * n_dense * (int64_t)sizeof(A))) {
-    return fail(FSP_ERR_WORKSPACE, "fsp_spmm_csr: partials need %lld bytes",
-                (long long)(plan->n_segments * n_dense * (int64_t)sizeof(A)));
+  const int64_t partial_bytes = plan->n_segments * x.n_dense * (int64_t)sizeof(A);
+  if (plan->n_segments > 0 && (plan->partials_dev == nullptr || plan->partials_bytes < partial_bytes)) {
+    return fail(FSP_ERR_WORKSPACE, "fsp_spmm_csr: partials need %lld bytes", (long long)partial_bytes);
   }
   A* partials = static_cast<A*>(plan->partials_dev);
   const int64_t ytiles = ceil_div(vcols, G);
-  if (ytiles > 65535) return fail(FSP_ERR_ARG, "fsp_spmm_csr: n_dense %lld too large", (long long)n_dense);
+  if (ytiles > 65535) return fail(FSP_ERR_ARG, "fsp_spmm_csr: n_dense %lld too large", (long long)x.n_dense);
   if (blocks > 0) {
     const dim3 grid(static_cast<unsigned>(blocks), static_cast<unsigned>(ytiles));
-#define FSP_SPMM_LAUNCH(U)                                                                                 \
-  hipLaunchKernelGGL((spmm_bins_kernel<T, IT, PT, QT, VEC, U, PERM>), grid, dim3(FSP_WG_THREADS), 0, s,     \
-                     static_cast<const PT*>(indptr), static_cast<const IT*>(indices),                       \
-                     static_cast<const T*>(data), static_cast<const QT*>(perm), static_cast<const T*>(B), ldb, \
-                     static_cast<T*>(C), ldc, n_dense, g_log2, partials, tab)
-    if (plan->unroll <= 2) {
-      FSP_SPMM_LAUNCH(2);
-    } else if (plan->unroll <= 4) {
-      FSP_SPMM_LAUNCH(4);
+    // 32-bit byte offsets when every array the kernel addresses fits in 4 GiB.
+    const int64_t lim = int64_t(1) << 32;
+    const bool off32 = std::is_same<PT, int32_t>::value && (!PERM || std::is_same<QT, int32_t>::value) &&
+                       x.nnz * (int64_t)sizeof(T) < lim && x.nnz * (int64_t)sizeof(IT) < lim &&
+                       x.nnz * (int64_t)sizeof(QT) < lim && x.b_rows * x.ldb * (int64_t)sizeof(T) < lim &&
+                       x.n_rows * x.ldc * (int64_t)sizeof(T) < lim && partial_bytes < lim;
+    if (off32) {
+      using P32 = int32_t;
+      using Q32 = int32_t;
+      if (plan->unroll >= 8) {
+        launch_bins<T, IT, P32, Q32, VEC, 8, PERM, true>(x, grid, g_log2, partials, tab, s);
+      } else {
+        launch_bins<T, IT, P32, Q32, VEC, 4, PERM, true>(x, grid, g_log2, partials, tab, s);
+      }
     } else {
-      FSP_SPMM_LAUNCH(8);
+      launch_bins<T, IT, PT, QT, VEC, 4, PERM, false>(x, grid, g_log2, partials, tab, s);
     }
-#undef FSP_SPMM_LAUNCH
     const int st = check_launch("spmm_bins");
     if (st) return st;
   }
   if (plan->n_long_rows > 0) {
-    const int64_t gx = ceil_div(n_dense, FSP_WG_THREADS);
-    int64_t gy = plan->n_long_rows;
-    if (gy > 65535) gy = 65535;
-    hipLaunchKernelGGL(spmm_long_reduce_kernel<T>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy)),
-                       dim3(FSP_WG_THREADS), 0, s, partials, plan->long_rows_dev, plan->long_seg_ptr_dev,
-                       plan->n_long_rows, n_dense, static_cast<T*>(C), ldc);
+    int64_t g = ceil_div(plan->n_long_rows * x.n_dense, FSP_WG_THREADS);
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(spmm_long_reduce_kernel<T>, dim3(static_cast<unsigned>(g)), dim3(FSP_WG_THREADS), 0, s,
+                       partials, plan->long_rows_dev, plan->long_seg_ptr_dev, plan->n_long_rows, x.n_dense,
+                       static_cast<T*>(x.C), x.ldc);
     return check_launch("spmm_long_reduce");
   }
   return FSP_OK;
@@ -178,16 +288,11 @@ static bool wide_ok(const void* B, int64_t ldb, const void* C, int64_t ldc, int6
 }
 
 template <typename T, typename IT, typename PT, typename QT, bool PERM>
-static int launch_spmm_vec(const void* indptr, const void* indices, const void* data, const void* perm,
-                           const void* B, int64_t ldb, void* C, int64_t ldc, int64_t n_dense, int lanes_n,
-                           const fsp_csr_plan* plan, hipStream_t s) {
+static int launch_spmm_vec(spmm_args x, const fsp_csr_plan* plan, hipStream_t s) {
   constexpr int V = 16 / sizeof(T);
-  if (wide_ok<T>(B, ldb, C, ldc, n_dense)) {
-    return launch_spmm<T, IT, PT, QT, V, PERM>(indptr, indices, data, perm, B, ldb, C, ldc, n_dense, lanes_n,
-                                               plan, s);
-  }
-  return launch_spmm<T, IT, PT, QT, 1, PERM>(indptr, indices, data, perm, B, ldb, C, ldc, n_dense,
-                                             0, plan, s);
+  if (wide_ok<T>(x.B, x.ldb, x.C, x.ldc, x.n_dense)) return launch_spmm<T, IT, PT, QT, V, PERM>(x, plan, s);
+  x.lanes_n = 0;
+  return launch_spmm<T, IT, PT, QT, 1, PERM>(x, plan, s);
 }
 
 }  // namespace fsp
@@ -197,8 +302,9 @@ using namespace fsp;
 extern "C" int fsp_spmm_csr(int32_t value_type, int32_t index_type, int32_t indptr_type, const void* indptr_dev,
                             const void* indices_dev, const void* data_dev, int32_t perm_type, const void* perm_dev,
                             const void* B_dev, int64_t ldb, void* C_dev, int64_t ldc, int64_t n_rows,
-                            int64_t n_dense, int32_t lanes_n, const fsp_csr_plan* plan, void* stream) {
-  if (n_rows < 0 || n_dense < 0) return fail(FSP_ERR_ARG, "fsp_spmm_csr: negative size");
+                            int64_t n_dense, int64_t nnz, int64_t b_rows, int32_t lanes_n,
+                            const fsp_csr_plan* plan, void* stream) {
+  if (n_rows < 0 || n_dense < 0 || nnz < 0 || b_rows < 0) return fail(FSP_ERR_ARG, "fsp_spmm_csr: negative size");
   int st = check_plan(plan, n_rows);
   if (st) return st;
   if (n_rows == 0 || n_dense == 0) return FSP_OK;
@@ -208,16 +314,29 @@ extern "C" int fsp_spmm_csr(int32_t value_type, int32_t index_type, int32_t indp
   if (ldb < n_dense || ldc < n_dense) return fail(FSP_ERR_ARG, "fsp_spmm_csr: leading dimension < n_dense");
   const hipStream_t s = as_stream(stream);
   const bool use_perm = perm_dev != nullptr;
+  spmm_args x{indptr_dev, indices_dev, data_dev, perm_dev, B_dev, ldb, C_dev, ldc, n_rows, n_dense, lanes_n, nnz, b_rows};
+  if (nnz == 0) {
+    // The kernels read clamped in-range indices; with no nonzeros there are
+    // none, so the all-zero result is written directly.
+    FSP_DISPATCH_VALUE(value_type, T, {
+      int64_t g = ceil_div(n_rows * n_dense, FSP_WG_THREADS);
+      if (g > 65536) g = 65536;
+      hipLaunchKernelGGL(fill_zero_rows_kernel<T>, dim3(static_cast<unsigned>(g)), dim3(FSP_WG_THREADS), 0, s,
+                         static_cast<T*>(C_dev), ldc, n_rows, n_dense);
+      return check_launch("fill_zero_rows");
+    });
+  }
+  if (indices_dev == nullptr || data_dev == nullptr || b_rows == 0) {
+    return fail(FSP_ERR_ARG, "fsp_spmm_csr: NULL indices/data or empty B with nonzeros");
+  }
   FSP_DISPATCH_VALUE(value_type, T, {
     FSP_DISPATCH_INDEX(index_type, IT, {
       FSP_DISPATCH_INDEX(indptr_type, PT, {
         if (!use_perm) {
-          return launch_spmm_vec<T, IT, PT, int32_t, false>(indptr_dev, indices_dev, data_dev, nullptr, B_dev, ldb,
-                                                            C_dev, ldc, n_dense, lanes_n, plan, s);
+          return launch_spmm_vec<T, IT, PT, int32_t, false>(x, plan, s);
         }
         FSP_DISPATCH_INDEX(perm_type, QT, {
-          return launch_spmm_vec<T, IT, PT, QT, true>(indptr_dev, indices_dev, data_dev, perm_dev, B_dev, ldb,
-                                                      C_dev, ldc, n_dense, lanes_n, plan, s);
+          return launch_spmm_vec<T, IT, PT, QT, true>(x, plan, s);
         });
       });
     });

@@ -17,6 +17,13 @@ Workloads (sizes of the headline benchmarks):
                            deduplicated, no self loops), values
                            D^-1/2 A D^-1/2 (200000 nodes, exponent 2.3,
                            mean degree 14).
+* ``bsr_blocks``        -- BSR with distinct uniform block columns per block
+                           row and N(0, 1) entries (65536 x 65536, 16 x 16,
+                           40 blocks per block row).
+* ``csr_batch``         -- a packed batch of square CSR matrices: rows
+                           log-uniform, row lengths uniform, distinct sorted
+                           columns one per stratum (4096 matrices, 64..8192
+                           rows, 4..64 nonzeros per row).
 """
 
 import numpy as np
@@ -148,3 +155,100 @@ def to_torch(indptr, indices, values, device, value_dtype=torch.float32, index_d
 
 def dense(array, device, dtype=torch.float32):
     return torch.from_numpy(np.ascontiguousarray(array)).to(device=device, dtype=dtype)
+
+
+def bsr_blocks(n_block_rows=4096, n_block_cols=4096, per_row=40, block=16, seed=2):
+    """(block_ptr, block_col, block_data[nnzb, block, block]) with `per_row`
+    distinct uniform block columns per block row (ascending) and N(0, 1)
+    entries (BSR SpMM 65536 x 65536, 16 x 16 blocks, 40 per block row).
+    `per_row` may also be an array of per-block-row counts."""
+    rng = _rng(seed)
+    lengths = np.broadcast_to(np.asarray(per_row, dtype=np.int64), (n_block_rows,))
+    block_ptr, block_col = distinct_sorted_columns(rng, lengths, n_block_cols)
+    block_data = rng.standard_normal((block_col.size, block, block))
+    return block_ptr, block_col, block_data
+
+
+def bsr_to_csr(block_ptr, block_col, block_data, shape=None):
+    """Expand a BSR matrix into canonical CSR.  Every block entry is kept
+    (zeros included); with `shape`, rows >= shape[0] and columns >= shape[1]
+    (the padding of the last block row / column) are dropped."""
+    block_ptr = np.asarray(block_ptr, dtype=np.int64)
+    block_col = np.asarray(block_col, dtype=np.int64)
+    bs = block_data.shape[1]
+    nbr = block_ptr.size - 1
+    per = np.diff(block_ptr)
+    n_rows_full = nbr * bs
+    indptr = np.zeros(n_rows_full + 1, dtype=np.int64)
+    np.cumsum(np.repeat(per * bs, bs), out=indptr[1:])
+    indices = np.empty(int(indptr[-1]), dtype=np.int64)
+    values = np.empty(int(indptr[-1]), dtype=block_data.dtype)
+    lane = np.arange(bs, dtype=np.int64)
+    for br in range(nbr):
+        b0, b1 = int(block_ptr[br]), int(block_ptr[br + 1])
+        if b0 == b1:
+            continue
+        cols = (block_col[b0:b1, None] * bs + lane[None, :]).reshape(-1)      # one row's columns
+        vals = block_data[b0:b1].transpose(1, 0, 2).reshape(bs, -1)          # [row in block, cols]
+        p0 = int(indptr[br * bs])
+        width = cols.size
+        indices[p0:p0 + bs * width] = np.tile(cols, bs)
+        values[p0:p0 + bs * width] = vals.reshape(-1)
+    if shape is None:
+        return indptr, indices, values
+    n_rows, n_cols = int(shape[0]), int(shape[1])
+    row_of = np.repeat(np.arange(n_rows_full, dtype=np.int64), np.diff(indptr))
+    keep = (row_of < n_rows) & (indices < n_cols)
+    lengths = np.bincount(row_of[keep], minlength=n_rows_full)[:n_rows]
+    out_ptr = np.zeros(n_rows + 1, dtype=np.int64)
+    np.cumsum(lengths, out=out_ptr[1:])
+    return out_ptr, indices[keep], values[keep]
+
+
+def csr_batch(n_mats=4096, rows_lo=64, rows_hi=8192, nnz_lo=4, nnz_hi=64, seed=4, device="cpu"):
+    """Packed batch of square CSR matrices (float32 values U[-1, 1], int32
+    indices), generated with torch's seeded generator on `device` (the full
+    4096-matrix batch has ~2.4e8 nonzeros: build it on the GPU).
+
+    Matrix sizes n = floor(exp(U[ln rows_lo, ln(rows_hi + 1)))) clipped to
+    [rows_lo, rows_hi]; every row's length is uniform in [nnz_lo, nnz_hi]
+    (clipped to n); a row of length L over n columns takes one uniform
+    column from each of the L integer strata [floor(j n / L),
+    floor((j + 1) n / L)), so its columns are distinct and ascending without
+    a sort.  Returns (indptr int32, indices int32, values f32, row_offsets,
+    nnz_offsets) as tensors on `device`: the local row pointers of all
+    matrices concatenated (matrix m's at row_offsets[m] + m), local columns,
+    values, and the int64 offset tables."""
+    dev = torch.device(device)
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    i64 = dict(dtype=torch.int64, device=dev)
+    u = torch.rand(n_mats, generator=g, dtype=torch.float64, device=dev)
+    lo_l, hi_l = float(np.log(rows_lo)), float(np.log(rows_hi + 1))
+    n = torch.floor(torch.exp(lo_l + u * (hi_l - lo_l))).to(torch.int64).clamp_(rows_lo, rows_hi)
+    row_offsets = torch.zeros(n_mats + 1, **i64)
+    torch.cumsum(n, 0, out=row_offsets[1:])
+    total_rows = int(row_offsets[-1])
+    mat_of_row = torch.repeat_interleave(torch.arange(n_mats, **i64), n)
+    lengths = torch.randint(nnz_lo, nnz_hi + 1, (total_rows,), generator=g, **i64)
+    lengths = torch.minimum(lengths, n[mat_of_row])
+    csum = torch.cumsum(lengths, 0)
+    nnz_offsets = torch.zeros(n_mats + 1, **i64)
+    nnz_offsets[1:] = csum[row_offsets[1:] - 1]
+    total = int(nnz_offsets[-1])
+    indptr = torch.zeros(total_rows + n_mats, dtype=torch.int32, device=dev)
+    indptr[torch.arange(total_rows, **i64) + mat_of_row + 1] = (csum - nnz_offsets[mat_of_row]).to(torch.int32)
+    row_start = csum - lengths
+    owner = torch.repeat_interleave(torch.arange(total_rows, **i64), lengths)
+    j = torch.arange(total, **i64) - row_start[owner]
+    L = lengths[owner]
+    nr = n[mat_of_row[owner]]
+    del owner
+    lo = torch.div(j * nr, L, rounding_mode="floor")
+    hi = torch.div((j + 1) * nr, L, rounding_mode="floor")
+    del j, L, nr
+    pick = torch.rand(total, generator=g, dtype=torch.float64, device=dev)
+    indices = (lo + torch.floor(pick * (hi - lo)).to(torch.int64)).to(torch.int32)
+    del lo, hi, pick
+    values = torch.rand(total, generator=g, dtype=torch.float32, device=dev) * 2.0 - 1.0
+    return indptr, indices, values, row_offsets, nnz_offsets

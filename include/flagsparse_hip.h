@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSP_ABI_VERSION 1
+#define FSP_ABI_VERSION 3
 #define FSP_MAX_BINS 8
 
 typedef enum fsp_value_type {
@@ -74,6 +74,9 @@ typedef struct fsp_csr_plan {
   const int64_t* long_seg_ptr_dev;/* [n_long_rows+1] segment range per long row */
   void* partials_dev;             /* scratch: n_segments * dense_cols * acc bytes */
   int64_t partials_bytes;
+  int64_t max_blocks;             /* 0: one workgroup per row tile; > 0: persistent
+                                     grid of at most this many workgroups, each
+                                     walking a contiguous run of tiles */
 } fsp_csr_plan;
 
 /* ---- library ------------------------------------------------------------ */
@@ -137,13 +140,15 @@ int fsp_spmv_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
  * _triton_spmm_csr_impl (spmm_csr.py:3373-3466), and the bucketed /
  * long-row split kernels of the opt path (spmm_csr.py:3469-3794).
  * Accumulation: f32 for f32/f16/bf16, f64 for f64 (spmm_csr.py:3419).
- * `lanes_n` = lanes spanning the dense dimension (power of two, 1..64). */
+ * `lanes_n` = lanes spanning the dense dimension (power of two, 1..64);
+ * `nnz` = nonzeros of A, `b_rows` = rows of B (they bound the kernel's
+ * offsets: 32-bit addressing is used when every array fits in 4 GiB). */
 int fsp_spmm_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
                  const void* indptr_dev, const void* indices_dev,
                  const void* data_dev, int32_t perm_type, const void* perm_dev,
                  const void* B_dev, int64_t ldb, void* C_dev, int64_t ldc,
-                 int64_t n_rows, int64_t n_dense, int32_t lanes_n,
-                 const fsp_csr_plan* plan, void* stream);
+                 int64_t n_rows, int64_t n_dense, int64_t nnz, int64_t b_rows,
+                 int32_t lanes_n, const fsp_csr_plan* plan, void* stream);
 
 /* ---- SDDMM ---------------------------------------------------------------
  * out[p] = alpha * <X[row(p), :], Y[col(p), :]> + beta * data_in[p]
@@ -151,13 +156,72 @@ int fsp_spmm_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
  * Y is [n_cols, k] (row stride ldy).  Replaces _sddmm_csr_real_kernel
  * (sddmm_csr.py:121-164) launched by _run_sddmm_prepared (sddmm_csr.py:291-386).
  * Accumulation f32 (f32/f16/bf16) or f64 (f64).  The plan's segments cut long
- * rows; SDDMM needs no partials. */
+ * rows; SDDMM needs no partials.  `nnz` = nonzeros of the pattern, `y_rows` =
+ * rows of Y (bounds for 32-bit addressing). */
 int fsp_sddmm_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
                   const void* indptr_dev, const void* indices_dev,
                   const void* X_dev, int64_t ldx, const void* Y_dev, int64_t ldy,
                   const void* data_in_dev, void* out_dev, int64_t n_rows,
-                  int64_t k, double alpha, double beta, int32_t lanes_k,
-                  const fsp_csr_plan* plan, void* stream);
+                  int64_t k, int64_t nnz, int64_t y_rows, double alpha, double beta,
+                  int32_t lanes_k, const fsp_csr_plan* plan, void* stream);
+
+/* ---- BSR (16 x 16 blocks) on the matrix cores ------------------------------
+ * No reference counterpart computes BSR SpMM; the block storage follows the
+ * reference's BSR SpMV (spmv_bsr.py:448-470: data[nnzb][bd][bd] row-major
+ * inside a block, block columns ascending inside a block row).
+ *
+ * CSR -> BSR (bit-exact vs a host conversion): fsp_csr_to_bsr_count sorts the
+ * block keys and writes the block count to *nnzb_dev; the caller reads it,
+ * allocates the outputs and calls fsp_csr_to_bsr_fill with the same
+ * workspace.  Values of a block that no nonzero touches are zero. */
+int fsp_csr_to_bsr_workspace(int64_t n_rows, int64_t n_cols, int64_t nnz, int64_t* bytes);
+int fsp_csr_to_bsr_count(int32_t indptr_type, int32_t index_type, const void* indptr_dev,
+                         const void* indices_dev, int64_t n_rows, int64_t n_cols, int64_t nnz,
+                         void* ws_dev, int64_t ws_bytes, int64_t* nnzb_dev, void* stream);
+int fsp_csr_to_bsr_fill(int32_t value_type, int32_t indptr_type, int32_t index_type,
+                        const void* indptr_dev, const void* indices_dev, const void* data_dev,
+                        int64_t n_rows, int64_t n_cols, int64_t nnz, int64_t nnzb,
+                        int32_t block_ptr_type, void* block_ptr_dev, int32_t* block_col_dev,
+                        void* block_data_dev, void* ws_dev, int64_t ws_bytes, void* stream);
+/* max |x| over a [rows, cols] array (row stride ld) into *out_dev (float). */
+int fsp_absmax(int32_t value_type, const void* x_dev, int64_t rows, int64_t cols, int64_t ld,
+               float* out_dev, void* stream);
+/* Packs B [K, N] (row stride ldb) into the MFMA operand layout
+ * Bp[k/8][n][k%8] of out_type (FSP_BF16 / FSP_F16, 16 bytes per (k/8, n);
+ * FSP_F8E4M3, 8 bytes, codes of B * 448 / absmax).  Bp holds
+ * ceil(K/16)*2 x ceil(N/128)*128 entries of 8 elements; padding is zero. */
+int fsp_bsr_pack_b(int32_t in_type, int32_t out_type, const void* B_dev, int64_t ldb, int64_t K,
+                   int64_t N, const float* absmax_dev, void* Bp_dev, void* stream);
+/* Per-block absmax quantisation to OCP FP8 e4m3: codes[nnzb][16][16] and
+ * scales[nnzb] with block = codes * scale (round to nearest even). */
+int fsp_bsr_quantize_fp8(int32_t value_type, const void* block_data_dev, int64_t nnzb, uint8_t* q_dev,
+                         float* scales_dev, void* stream);
+/* C[M, N] (f32, row stride ldc) = BSR(A) @ B with B packed by fsp_bsr_pack_b.
+ * value_type FSP_BF16 / FSP_F16 (block_data of that type) or FSP_F8E4M3
+ * (block_data = codes, block_scales and the B absmax used for packing). */
+int fsp_bsr_spmm(int32_t value_type, int32_t block_ptr_type, const void* block_ptr_dev,
+                 const int32_t* block_col_dev, const void* block_data_dev, const float* block_scales_dev,
+                 int64_t n_block_rows, int64_t nnzb, const void* Bp_dev, const float* b_absmax_dev,
+                 int64_t K, int64_t N, float* C_dev, int64_t ldc, int64_t M, void* stream);
+
+/* ---- batches of CSR matrices --------------------------------------------
+ * No counterpart in the reference (it runs one matrix per call).  A batch of
+ * n_mats matrices is a packed offset table over concatenated arrays: matrix m
+ * has rows [row_off[m], row_off[m+1]), columns [col_off[m], col_off[m+1]) and
+ * nonzeros [nnz_off[m], nnz_off[m+1]); its own row pointer (local offsets from
+ * 0, n_m + 1 entries) sits at local_indptr[row_off[m] + m].  fsp_batch_pack
+ * writes the block-diagonal CSR of the batch: out_indptr [total_rows + 1]
+ * (global offsets) and out_indices [total_nnz] (col_off[m] + local column).
+ * The values are shared.  Violations (row pointer not starting at 0 / ending
+ * at the matrix's nonzero count: bit 1; decreasing: bit 2; a column outside
+ * its matrix: bit 4) are OR-ed into *err_dev.  The block-diagonal matrix then
+ * runs through fsp_spmv_csr / fsp_spmm_csr as ONE launch for the batch
+ * (plan.max_blocks > 0 makes it persistent). */
+int fsp_batch_pack(int32_t indptr_type, int32_t index_type, int64_t n_mats, const int64_t* row_off_dev,
+                   const int64_t* col_off_dev, const int64_t* nnz_off_dev, const void* local_indptr_dev,
+                   const void* local_indices_dev, int64_t total_rows, int64_t total_nnz, int32_t out_indptr_type,
+                   void* out_indptr_dev, int32_t out_index_type, void* out_indices_dev, uint32_t* err_dev,
+                   void* stream);
 
 #ifdef __cplusplus
 }

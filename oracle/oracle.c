@@ -118,3 +118,64 @@ void orc_csr_to_csc(int64_t n_rows, int64_t n_cols, const int64_t* indptr, const
   }
   free(next);
 }
+
+/* CSR -> BSR with bs x bs blocks (no reference counterpart on this path; the
+ * storage convention data[nnzb][bs][bs], row-major inside a block, block
+ * columns ascending inside a block row, follows the reference's BSR SpMV,
+ * spmv_bsr.py:451-452).  Pass block_col == NULL to only count: returns nnzb.
+ * block_data must be zeroed by the caller (missing entries are zero). */
+int64_t orc_csr_to_bsr(int64_t n_rows, int64_t n_cols, int64_t bs, const int64_t* indptr, const int64_t* indices,
+                       const double* data, int64_t* block_ptr, int64_t* block_col, double* block_data) {
+  const int64_t nbr = (n_rows + bs - 1) / bs, nbc = (n_cols + bs - 1) / bs;
+  int64_t* mark = (int64_t*)malloc((size_t)(nbc > 0 ? nbc : 1) * sizeof(int64_t));
+  int64_t* slot = (int64_t*)malloc((size_t)(nbc > 0 ? nbc : 1) * sizeof(int64_t));
+  for (int64_t c = 0; c < nbc; ++c) mark[c] = -1;
+  int64_t nnzb = 0;
+  if (block_ptr) block_ptr[0] = 0;
+  for (int64_t br = 0; br < nbr; ++br) {
+    const int64_t r0 = br * bs, r1 = (r0 + bs < n_rows) ? r0 + bs : n_rows;
+    /* collect the distinct block columns of this block row, ascending */
+    int64_t cnt = 0;
+    for (int64_t r = r0; r < r1; ++r)
+      for (int64_t p = indptr[r]; p < indptr[r + 1]; ++p) {
+        const int64_t bc = indices[p] / bs;
+        if (mark[bc] != br) { mark[bc] = br; ++cnt; }
+      }
+    if (block_col) {
+      int64_t k = nnzb;
+      for (int64_t bc = 0; bc < nbc; ++bc)
+        if (mark[bc] == br) { block_col[k] = bc; slot[bc] = k; ++k; }
+      for (int64_t r = r0; r < r1; ++r)
+        for (int64_t p = indptr[r]; p < indptr[r + 1]; ++p) {
+          const int64_t bc = indices[p] / bs;
+          block_data[(slot[bc] * bs + (r - r0)) * bs + (indices[p] - bc * bs)] += data[p];
+        }
+    }
+    nnzb += cnt;
+    if (block_ptr) block_ptr[br + 1] = nnzb;
+  }
+  free(mark);
+  free(slot);
+  return nnzb;
+}
+
+/* C[n_brows*bs, n_dense] = BSR(A) @ B, long-double accumulation. */
+void orc_bsr_spmm(int64_t n_brows, int64_t bs, int64_t n_dense, const int64_t* block_ptr, const int64_t* block_col,
+                  const double* block_data, const double* B, double* C) {
+  long double* acc = (long double*)malloc((size_t)(bs * (n_dense > 0 ? n_dense : 1)) * sizeof(long double));
+  for (int64_t br = 0; br < n_brows; ++br) {
+    for (int64_t i = 0; i < bs * n_dense; ++i) acc[i] = 0.0L;
+    for (int64_t j = block_ptr[br]; j < block_ptr[br + 1]; ++j) {
+      const double* blk = block_data + j * bs * bs;
+      for (int64_t r = 0; r < bs; ++r)
+        for (int64_t c = 0; c < bs; ++c) {
+          const long double a = (long double)blk[r * bs + c];
+          if (a == 0.0L) continue;
+          const double* brow = B + (block_col[j] * bs + c) * n_dense;
+          for (int64_t n = 0; n < n_dense; ++n) acc[r * n_dense + n] += a * (long double)brow[n];
+        }
+    }
+    for (int64_t i = 0; i < bs * n_dense; ++i) C[br * bs * n_dense + i] = (double)acc[i];
+  }
+  free(acc);
+}

@@ -47,8 +47,12 @@ def _load():
         lib.orc_sddmm_csr.argtypes = [_c_i64, _c_i64, _i64p, _i64p, _f64p, _f64p, ctypes.c_void_p, _c_dbl, _c_dbl,
                                       _f64p]
         lib.orc_csr_to_csc.argtypes = [_c_i64, _c_i64, _i64p, _i64p, _i64p, _i64p, _i64p]
+        lib.orc_csr_to_bsr.argtypes = [_c_i64, _c_i64, _c_i64, _i64p, _i64p, _f64p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p]
+        lib.orc_csr_to_bsr.restype = _c_i64
+        lib.orc_bsr_spmm.argtypes = [_c_i64, _c_i64, _c_i64, _i64p, _i64p, _f64p, _f64p, _f64p]
         for fn in (lib.orc_spmv_csr, lib.orc_spmv_csr_trans, lib.orc_spmm_csr, lib.orc_spmm_csr_trans,
-                   lib.orc_sddmm_csr, lib.orc_csr_to_csc):
+                   lib.orc_sddmm_csr, lib.orc_csr_to_csc, lib.orc_bsr_spmm):
             fn.restype = None
         _lib = lib
     return _lib
@@ -114,6 +118,101 @@ def csr_to_csc(indptr, indices, shape):
     perm = np.empty(indices.size, dtype=np.int64)
     _load().orc_csr_to_csc(n_rows, n_cols, indptr, indices, colptr, rowind, perm)
     return colptr, rowind, perm
+
+
+def csr_to_bsr(indptr, indices, data, shape, bs=16):
+    """(block_ptr, block_col, block_data[nnzb, bs, bs]) with block columns
+    ascending inside each block row; the block storage convention of the
+    reference's BSR SpMV (spmv_bsr.py:451-452)."""
+    n_rows, n_cols = int(shape[0]), int(shape[1])
+    indptr, indices, data = _i64(indptr), _i64(indices), _f64(data)
+    lib = _load()
+    nnzb = lib.orc_csr_to_bsr(n_rows, n_cols, bs, indptr, indices, data, None, None, None)
+    nbr = -(-n_rows // bs)
+    block_ptr = np.zeros(nbr + 1, dtype=np.int64)
+    block_col = np.zeros(max(nnzb, 1), dtype=np.int64)
+    block_data = np.zeros((max(nnzb, 1), bs, bs), dtype=np.float64)
+    lib.orc_csr_to_bsr(n_rows, n_cols, bs, indptr, indices, data, block_ptr.ctypes.data, block_col.ctypes.data,
+                       block_data.ctypes.data)
+    return block_ptr, block_col[:nnzb], block_data[:nnzb]
+
+
+def bsr_spmm(block_ptr, block_col, block_data, B, bs=16):
+    """C = BSR(A) @ B with long-double accumulation."""
+    block_ptr, block_col = _i64(block_ptr), _i64(block_col)
+    data = _f64(np.asarray(block_data).reshape(-1))
+    B = _f64(B)
+    nbr = block_ptr.size - 1
+    C = np.empty((nbr * bs, B.shape[1]))
+    _load().orc_bsr_spmm(nbr, bs, int(B.shape[1]), block_ptr, block_col, data, B, C)
+    return C
+
+
+# ---- OCP FP8 E4M3 (finite-only "fn" variant): bias 7, 3 mantissa bits,
+# largest finite 448, no infinities, 0x7F/0xFF = NaN.  Restated from the
+# Open Compute Project OFP8 specification (round to nearest, ties to even).
+E4M3_MAX = 448.0
+
+
+def e4m3_encode(x):
+    """float array -> uint8 E4M3FN codes (RNE; |x| > 448 saturates to 448)."""
+    x = np.asarray(x, dtype=np.float64)
+    sign = (np.signbit(x)).astype(np.uint8) << 7
+    a = np.minimum(np.abs(x), E4M3_MAX)
+    code = np.zeros(a.shape, dtype=np.int64)
+    normal = a >= 2.0 ** -6
+    # subnormals: value = m * 2^-9, m in 0..8 (m == 8 carries into the first normal)
+    m_sub = np.rint(a / 2.0 ** -9).astype(np.int64)
+    code = np.where(normal, 0, m_sub)
+    e = np.floor(np.log2(np.where(normal, a, 1.0))).astype(np.int64)
+    mant = np.rint(np.where(normal, a, 1.0) / np.exp2(e - 3)).astype(np.int64)  # 8..16
+    carry = mant == 16
+    e = np.where(carry, e + 1, e)
+    mant = np.where(carry, 8, mant)
+    code = np.where(normal, ((e + 7) << 3) | (mant - 8), code)
+    return (code.astype(np.uint8) | sign).astype(np.uint8)
+
+
+def e4m3_decode(code):
+    code = np.asarray(code, dtype=np.uint8)
+    sign = np.where(code & 0x80, -1.0, 1.0)
+    e = (code >> 3) & 0xF
+    m = (code & 0x7).astype(np.float64)
+    val = np.where(e == 0, m * 2.0 ** -9, (1.0 + m / 8.0) * np.exp2(e.astype(np.float64) - 7))
+    val = np.where((code & 0x7F) == 0x7F, np.nan, val)
+    return sign * val
+
+
+def fp8_block_quantize(block_data):
+    """Per-block absmax scaling to E4M3: scale = absmax / 448 (1 for an
+    all-zero block), codes = e4m3(block / scale)."""
+    b = np.asarray(block_data, dtype=np.float64)
+    absmax = np.abs(b.reshape(b.shape[0], -1)).max(axis=1) if b.size else np.zeros(0)
+    scale = np.where(absmax > 0, absmax / E4M3_MAX, 1.0)
+    codes = e4m3_encode(b / scale[:, None, None])
+    return codes, scale
+
+
+def fp8_block_quantize_f32(block_data):
+    """fp8_block_quantize in float32 arithmetic, as the device computes it:
+    scale = f32(absmax) / 448 (IEEE f32 division), codes = e4m3(f32(x) / scale)."""
+    b = np.asarray(block_data, dtype=np.float32)
+    n = b.shape[0]
+    absmax = np.abs(b.reshape(n, -1)).max(axis=1) if b.size else np.zeros(0, np.float32)
+    scale = np.where(absmax > 0, absmax / np.float32(E4M3_MAX), np.float32(1.0)).astype(np.float32)
+    x = (b / scale[:, None, None]).astype(np.float32)
+    return e4m3_encode(x.astype(np.float64)), scale
+
+
+def fp8_tensor_quantize_f32(x):
+    """One absmax scale for a whole tensor, float32 arithmetic:
+    codes = e4m3(f32(x) * f32(448 / absmax)), scale = f32(absmax / 448)."""
+    a = np.asarray(x, dtype=np.float32)
+    am = np.float32(np.abs(a).max()) if a.size else np.float32(0.0)
+    inv = np.float32(E4M3_MAX) / am if am > 0 else np.float32(1.0)
+    codes = e4m3_encode((a * np.float32(inv)).astype(np.float32).astype(np.float64))
+    scale = am / np.float32(E4M3_MAX) if am > 0 else np.float32(1.0)
+    return codes, np.float32(scale)
 
 
 def max_relative_error(candidate, reference):

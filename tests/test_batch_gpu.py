@@ -1,0 +1,220 @@
+"""GPU parity of the batched SpMV / SpMM (one launch for a packed batch).
+
+* every matrix of a batch against the FP64 oracle (f32 rel 1e-5, f64 1e-12);
+* bitwise equality with the per-matrix launch loop (same parameters: every
+  row is reduced in the same order);
+* persistent grids (capped workgroups, contiguous tile runs) bitwise equal to
+  one workgroup per tile;
+* HIP graph replays, each checked against the eager result, with eager work
+  and host syncs between replays;
+* validation of the packed offset table, the autotuner and its cache;
+* the full 4096-matrix batch (log-uniform 64..8192 rows, 4..64 nonzeros per
+  row) against the per-matrix loop (bitwise) and sampled matrices against the
+  oracle.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from flagsparse_amd import autotune
+from flagsparse_amd import synthetic as S
+from flagsparse_amd.batch import CsrBatch
+from oracle import oracle as O
+
+from ._helpers import REL_TOL, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _no_tune_file(monkeypatch):
+    monkeypatch.setenv("FLAGSPARSE_AMD_TUNE_CACHE", "")  # in-memory cache only
+    autotune.clear_memory()
+
+
+def _small_batch(seed, n_mats=23, max_rows=200, max_len=40, long_rows=(), rect=True):
+    """Per-matrix numpy CSRs: ragged sizes, empty rows, optional empty
+    matrices and rectangular shapes, plus optional long rows."""
+    rng = np.random.default_rng(seed)
+    long_of = {r % n_mats: L for r, L in long_rows}
+    mats = []
+    for m in range(n_mats):
+        n_rows = int(rng.integers(0, max_rows)) if m % 7 else 0
+        if m in long_of:
+            n_rows = max(n_rows, 1)
+        n_cols = int(rng.integers(1, max_rows + 1)) if rect else max(n_rows, 1)
+        lengths = rng.integers(0, min(max_len, n_cols) + 1, n_rows)
+        if m in long_of:
+            n_cols = max(n_cols, long_of[m])
+            lengths[0] = long_of[m]
+        ip, ix = S.distinct_sorted_columns(rng, lengths, n_cols)
+        v = rng.uniform(-1, 1, ix.size)
+        mats.append((ip, ix, v, (n_rows, n_cols)))
+    return mats
+
+
+def _to_batch(mats, device, dtype=torch.float32, index_dtype=torch.int32):
+    dev = [(S.dense(v, device, dtype), torch.from_numpy(ix).to(device, index_dtype),
+            torch.from_numpy(ip).to(device, index_dtype), shape) for ip, ix, v, shape in mats]
+    return CsrBatch.from_matrices(dev)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("index_dtype", [torch.int32, torch.int64])
+def test_batch_spmv_matches_oracle_and_loop(device, dtype, index_dtype):
+    mats = _small_batch(1)
+    b = _to_batch(mats, device, dtype, index_dtype)
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, b.total_cols)
+    xt = S.dense(x, device, dtype)
+    y = b.spmv(xt)
+    for m, (ip, ix, v, shape) in enumerate(mats):
+        want = O.spmv(ip, ix, v, x[b.col_offsets[m]: b.col_offsets[m + 1]], shape)
+        got = y[b.row_offsets[m]: b.row_offsets[m + 1]]
+        assert rel_err(got, want) <= REL_TOL[dtype], f"matrix {m}"
+    assert torch.equal(b.spmv_per_matrix(xt), y)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("k", [1, 32, 70])
+def test_batch_spmm_matches_oracle_and_loop(device, dtype, k):
+    mats = _small_batch(3)
+    b = _to_batch(mats, device, dtype)
+    B = np.random.default_rng(4).standard_normal((b.total_cols, k))
+    Bt = S.dense(B, device, dtype)
+    C = b.spmm(Bt)
+    for m, (ip, ix, v, shape) in enumerate(mats):
+        want = O.spmm(ip, ix, v, B[b.col_offsets[m]: b.col_offsets[m + 1]], shape)
+        assert rel_err(C[b.row_offsets[m]: b.row_offsets[m + 1]], want) <= REL_TOL[dtype], f"matrix {m}"
+    assert torch.equal(b.spmm_per_matrix(Bt), C)
+
+
+def test_batch_long_rows_split_into_segments(device):
+    # rows above the SpMV (4096) and SpMM (256) long thresholds
+    mats = _small_batch(5, n_mats=9, long_rows=((1, 9000), (4, 300), (6, 5000)))
+    b = _to_batch(mats, device)
+    x = np.random.default_rng(6).uniform(-1, 1, b.total_cols)
+    B = np.random.default_rng(7).standard_normal((b.total_cols, 16))
+    y = b.spmv(S.dense(x, device))
+    C = b.spmm(S.dense(B, device))
+    assert torch.equal(b.spmv_per_matrix(S.dense(x, device)), y)
+    assert torch.equal(b.spmm_per_matrix(S.dense(B, device)), C)
+    for m, (ip, ix, v, shape) in enumerate(mats):
+        r0, r1 = b.row_offsets[m], b.row_offsets[m + 1]
+        c0, c1 = b.col_offsets[m], b.col_offsets[m + 1]
+        assert rel_err(y[r0:r1], O.spmv(ip, ix, v, x[c0:c1], shape)) <= 1e-5
+        assert rel_err(C[r0:r1], O.spmm(ip, ix, v, B[c0:c1], shape)) <= 1e-5
+
+
+@pytest.mark.parametrize("bpc", [1, 2, 8])
+def test_persistent_grid_is_bitwise_equal(device, bpc):
+    ip, ix, v, ro, no = S.csr_batch(n_mats=96, rows_lo=16, rows_hi=2048, seed=11, device=device)
+    b = CsrBatch(v, ix, ip, ro, None, no)
+    x = torch.rand(b.total_cols, device=device) - 0.5
+    B = torch.randn(b.total_cols, 32, device=device)
+    base_v = autotune.TuneParams(unroll=4, blocks_per_cu=0)
+    pers_v = autotune.TuneParams(unroll=4, blocks_per_cu=bpc)
+    assert torch.equal(b.spmv(x, params=base_v), b.spmv(x, params=pers_v))
+    base_m = autotune.TuneParams(unroll=4, slot_scale=2, blocks_per_cu=0)
+    pers_m = autotune.TuneParams(unroll=4, slot_scale=2, blocks_per_cu=bpc)
+    assert torch.equal(b.spmm(B, params=base_m), b.spmm(B, params=pers_m))
+
+
+def test_graph_replays_match_eager(device):
+    ip, ix, v, ro, no = S.csr_batch(n_mats=64, rows_lo=16, rows_hi=1024, seed=12, device=device)
+    b = CsrBatch(v, ix, ip, ro, None, no)
+    x = torch.empty(b.total_cols, device=device)
+    y = torch.empty(b.total_rows, device=device)
+    B = torch.empty(b.total_cols, 32, device=device)
+    C = torch.empty(b.total_rows, 32, device=device)
+    pv = autotune.TuneParams(unroll=4, blocks_per_cu=4)
+    pm = autotune.TuneParams(unroll=4, slot_scale=4, blocks_per_cu=8)
+    b._params.clear()
+    x.uniform_(-1, 1)
+    B.normal_()
+    gv = b._capture(lambda: b.spmv(x, out=y, params=pv))
+    gm = b._capture(lambda: b.spmm(B, out=C, params=pm))
+    for rep in range(3):
+        x.uniform_(-1, 1)  # eager work between replays
+        B.normal_()
+        torch.cuda.synchronize()
+        gv.replay()
+        gm.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(y, b.spmv(x, params=pv)), f"spmv replay {rep}"
+        assert torch.equal(C, b.spmm(B, params=pm)), f"spmm replay {rep}"
+
+
+def test_batch_validation(device):
+    mats = _small_batch(8, n_mats=4)
+    good = _to_batch(mats, device)
+    data, indices, indptr = good.data, good.local_indices, good.local_indptr
+    ro, co, no = good.row_offsets, good.col_offsets, good.nnz_offsets
+    owner = int(np.searchsorted(no, 0, side="right")) - 1  # matrix of nonzero 0
+    bad = indices.clone()
+    bad[0] = int(co[owner + 1] - co[owner])  # one past its matrix's last column
+    with pytest.raises(IndexError, match="out of range"):
+        CsrBatch(data, bad, indptr, ro, co, no)
+    with pytest.raises(ValueError, match="indptr length"):
+        CsrBatch(data, indices, indptr[:-1], ro, co, no)
+    wrong = no.copy()
+    wrong[1:] += 1
+    wrong[-1] = no[-1]
+    with pytest.raises(ValueError):
+        CsrBatch(data, indices, indptr, ro, co, wrong)
+    with pytest.raises(ValueError, match="nondecreasing"):
+        CsrBatch(data, indices, indptr, ro[::-1].copy(), co, no)
+    with pytest.raises(ValueError, match="x.shape"):
+        good.spmv(torch.zeros(good.total_cols + 1, device=device))
+
+
+def test_autotune_picks_and_caches(device, tmp_path, monkeypatch):
+    path = tmp_path / "tune.json"
+    monkeypatch.setenv("FLAGSPARSE_AMD_TUNE_CACHE", str(path))
+    ip, ix, v, ro, no = S.csr_batch(n_mats=128, rows_lo=64, rows_hi=4096, seed=13, device=device)
+    b = CsrBatch(v, ix, ip, ro, None, no)
+    p = b.params("spmv")
+    assert p in autotune.candidates("spmv")
+    assert len(b.tune_results) == len(autotune.candidates("spmv"))
+    assert path.exists() and "spmv|float32" in path.read_text()
+    autotune.clear_memory()
+    b2 = CsrBatch(v, ix, ip, ro, None, no)
+    assert b2.params("spmv") == p  # read back from the file, not re-tuned
+    assert not hasattr(b2, "tune_results")
+    x = torch.rand(b.total_cols, device=device)
+    assert torch.equal(b2.spmv(x), b.spmv_per_matrix(x, params=p))
+
+
+# ------------------------------------- full batch: 4096 matrices, seed 4
+@pytest.fixture(scope="module")
+def full_batch():
+    dev = torch.device("cuda:0")
+    ip, ix, v, ro, no = S.csr_batch(seed=4, device=dev)
+    return CsrBatch(v, ix, ip, ro, None, no), (ip, ix, v)
+
+
+def _oracle_sample(b, arrays, x, B, y, C):
+    ip, ix, v = (a.cpu().numpy() for a in arrays)
+    for m in (0, 1, 2047, 4095):
+        r0, r1 = int(b.row_offsets[m]), int(b.row_offsets[m + 1])
+        n0, n1 = int(b.nnz_offsets[m]), int(b.nnz_offsets[m + 1])
+        lp = ip[r0 + m: r1 + m + 1]
+        xs = x[r0:r1].cpu().numpy()
+        assert rel_err(y[r0:r1], O.spmv(lp, ix[n0:n1], v[n0:n1], xs, (r1 - r0, r1 - r0))) <= 1e-5
+        Bs = B[r0:r1].cpu().numpy()
+        assert rel_err(C[r0:r1], O.spmm(lp, ix[n0:n1], v[n0:n1], Bs, (r1 - r0, r1 - r0))) <= 1e-5
+
+
+def test_full_batch(device, full_batch):
+    b, arrays = full_batch
+    assert b.n_mats == 4096
+    x = torch.rand(b.total_cols, device=device) * 2 - 1
+    B = torch.randn(b.total_cols, 32, device=device)
+    pv = autotune.TuneParams(unroll=4, blocks_per_cu=8)
+    pm = autotune.TuneParams(unroll=4, slot_scale=4, blocks_per_cu=8)
+    y = b.spmv(x, params=pv)
+    C = b.spmm(B, params=pm)
+    assert torch.equal(b.spmv_per_matrix(x, params=pv), y)
+    assert torch.equal(b.spmm_per_matrix(B, params=pm), C)
+    _oracle_sample(b, arrays, x, B, y, C)

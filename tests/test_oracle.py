@@ -142,3 +142,96 @@ def test_planted_mistakes_are_caught():
     assert not np.allclose(wrong, A @ x)
     dropped = O.spmv(ip, ix, np.where(np.arange(v.size) == 0, 0.0, v), x, (30, 30))  # dropped term
     assert not np.allclose(dropped, A @ x)
+
+
+@pytest.mark.parametrize("m,n,density", [(40, 33, 0.2), (64, 64, 0.05), (1, 100, 0.5), (100, 17, 0.3), (7, 9, 0.0)])
+def test_csr_to_bsr_matches_scipy_tobsr(m, n, density):
+    ip, ix, v, A = _mat(m, n, density, seed=m + n)
+    bp, bc, bd = O.csr_to_bsr(ip, ix, v, (m, n), bs=16)
+    mp, np_ = -(-m // 16) * 16, -(-n // 16) * 16
+    Ap = sp.csr_matrix((v, ix, np.concatenate([ip, np.full(mp - m, ip[-1])])), shape=(mp, np_))
+    S = Ap.tobsr(blocksize=(16, 16))
+    S.sort_indices()
+    # scipy keeps explicit blocks only where entries exist, like the oracle
+    np.testing.assert_array_equal(bp, S.indptr)
+    np.testing.assert_array_equal(bc, S.indices)
+    np.testing.assert_array_equal(bd, S.data)
+
+
+def test_bsr_spmm_matches_dense():
+    from flagsparse_amd import synthetic as S
+
+    bp, bc, bd = S.bsr_blocks(6, 9, 3, seed=1)
+    rng = np.random.default_rng(1)
+    B = rng.standard_normal((9 * 16, 5))
+    dense = np.zeros((6 * 16, 9 * 16))
+    for br in range(6):
+        for j in range(bp[br], bp[br + 1]):
+            dense[br * 16:(br + 1) * 16, bc[j] * 16:(bc[j] + 1) * 16] = bd[j]
+    np.testing.assert_allclose(O.bsr_spmm(bp, bc, bd, B), dense @ B, rtol=1e-12, atol=1e-12)
+    ip, ix, v = S.bsr_to_csr(bp, bc, bd)
+    np.testing.assert_allclose(O.spmm(ip, ix, v, B, (96, 144)), dense @ B, rtol=1e-12, atol=1e-12)
+
+
+def test_e4m3_codec_matches_torch_and_spec():
+    import torch
+
+    x = np.concatenate([np.random.default_rng(0).standard_normal(20000) * 40, np.linspace(-448, 448, 4001),
+                        2.0 ** -np.arange(0, 12), [0.0, -0.0, 2 ** -9, 2 ** -10, 1.5 * 2 ** -9, 448.0, 464.0 - 1e-9]])
+    codes = O.e4m3_encode(x)
+    ref = torch.tensor(x, dtype=torch.float64).to(torch.float8_e4m3fn).view(torch.uint8).numpy()
+    np.testing.assert_array_equal(codes, ref)
+    # spec anchors: 448 = 0x7E, smallest subnormal 2^-9 = 0x01, 1.0 = 0x38
+    assert O.e4m3_encode([448.0, 2 ** -9, 1.0]).tolist() == [0x7E, 0x01, 0x38]
+    dec = O.e4m3_decode(np.arange(256, dtype=np.uint8))
+    assert np.isnan(dec[0x7F]) and np.isnan(dec[0xFF]) and dec[0x7E] == 448.0
+    finite = np.isfinite(dec)
+    # every finite code round-trips (including -0.0 -> 0x80)
+    np.testing.assert_array_equal(O.e4m3_encode(dec[finite]), np.arange(256, dtype=np.uint8)[finite])
+
+
+def test_fp8_block_quantize_scale_rule():
+    bd = np.random.default_rng(2).standard_normal((5, 16, 16))
+    bd[3] = 0.0
+    codes, scale = O.fp8_block_quantize(bd)
+    np.testing.assert_allclose(scale[[0, 1, 2, 4]], np.abs(bd[[0, 1, 2, 4]]).reshape(4, -1).max(1) / 448.0)
+    assert scale[3] == 1.0 and not codes[3].any()
+    deq = O.e4m3_decode(codes) * scale[:, None, None]
+    # e4m3 keeps 3 mantissa bits: relative error <= 2^-4 for normals
+    big = np.abs(bd) > 2 ** -6 * scale[:, None, None]
+    assert np.all(np.abs(deq - bd)[big] <= 2 ** -4 * np.abs(bd)[big] + 1e-15)
+
+
+def test_bsr_to_csr_expansion_matches_scipy():
+    bp, bc, bd = S.bsr_blocks(7, 9, np.array([0, 3, 1, 9, 2, 0, 4]), seed=3)
+    A = sp.bsr_matrix((bd, bc, bp), shape=(112, 144)).toarray()
+    ip, ix, v = S.bsr_to_csr(bp, bc, bd)
+    C = sp.csr_matrix((v, ix, ip), shape=(112, 144))
+    assert C.has_sorted_indices and C.nnz == bd.size
+    np.testing.assert_array_equal(C.toarray(), A)
+    ip, ix, v = S.bsr_to_csr(bp, bc, bd, shape=(100, 130))
+    np.testing.assert_array_equal(sp.csr_matrix((v, ix, ip), shape=(100, 130)).toarray(), A[:100, :130])
+    # and back: the oracle's CSR -> BSR of the expansion is the identity
+    rp, rc, rd = O.csr_to_bsr(*S.bsr_to_csr(bp, bc, bd), (112, 144))
+    np.testing.assert_array_equal(rp, bp)
+    np.testing.assert_array_equal(rc, bc)
+    np.testing.assert_array_equal(rd, bd)
+
+
+def test_fp8_float32_quantizers():
+    rng = np.random.default_rng(3)
+    bd = rng.standard_normal((9, 16, 16)).astype(np.float32)
+    bd[2] = 0
+    c32, s32 = O.fp8_block_quantize_f32(bd)
+    c64, s64 = O.fp8_block_quantize(bd)
+    assert s32.dtype == np.float32
+    np.testing.assert_allclose(s32, s64, rtol=2 ** -23)
+    # the two roundings differ at most by one code step, on rare ties
+    assert np.mean(c32 == c64) > 0.99
+    deq = O.e4m3_decode(c32) * s32[:, None, None]
+    big = np.abs(bd) > 2 ** -6 * s32[:, None, None]
+    assert np.all(np.abs(deq - bd)[big] <= 2 ** -4 * np.abs(bd)[big] * (1 + 1e-6))
+    B = rng.standard_normal((30, 7)) * 5
+    cb, sb = O.fp8_tensor_quantize_f32(B)
+    assert sb == np.float32(np.abs(B.astype(np.float32)).max()) / np.float32(448)
+    assert np.abs(O.e4m3_decode(cb)).max() == 448.0
