@@ -13,6 +13,8 @@ the kernels run on.  The same line also carries the bf16-vs-fp32 fwd+bwd
 speedup of the differentiable SpMM (200000-node power-law graph, N=128) and
 the 16384 x 16384 x 16 SpMV rate, and a CPU baseline: the C oracle (one core)
 on a bounded slice of the same workload, plus torch.sparse on all host cores.
+Extras: the measured HBM copy rate, an HBM-bound 2^24-row SpMV, the BSR SpMM
+on the matrix cores (bf16 / f16 / fp8) and the 4096-matrix batched launch.
 """
 
 import argparse
@@ -160,6 +162,137 @@ def bench_autograd_bf16_vs_fp32(dev, steps, warmup):
     return out
 
 
+def bench_copy(dev, steps):
+    """Measured HBM copy rate (1 GiB device-to-device, read + write bytes);
+    its kernel also calibrates the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    n = 1 << 28  # 2^28 float32 = 1 GiB
+    src = torch.empty(n, dtype=torch.float32, device=dev).uniform_()
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream(dev)
+    _, ev = time_steps(lambda: dst.copy_(src), max(5, steps // 5), 2, stream)
+    med = statistics.median(ev)
+    del src, dst
+    return {"bytes": 2 * 4 * n, "event_ms_median": round(med, 5), "gbs": round(2 * 4 * n / (med * 1e-3) / 1e9, 2)}
+
+
+def bench_spmv_large(dev, steps, warmup):
+    """HBM-bound SpMV: 2^24 rows x 16384 columns, 16 distinct uniform columns
+    per row -- the 16384 x 16384 x 16 pattern with 1024x the rows, so the
+    index / value stream (2.2 GB) is far beyond the caches while x (64 KB)
+    stays cache-resident, as it does for the small matrix."""
+    m, n = 1 << 24, 16384
+    ptr, idx, data, x = synthetic.uniform_rows_device(m, n, 16, seed=0, device=dev)
+    handle = CsrHandle(idx, ptr, (m, n))
+    y = torch.empty(m, dtype=torch.float32, device=dev)
+    handle.spmv(data, x, out=y)
+    stream = torch.cuda.current_stream(dev)
+    _, ev = time_steps(lambda: handle.spmv(data, x, out=y), steps, warmup, stream)
+    med = statistics.median(ev)
+    nnz = m * 16
+    nbytes = nnz * 8 + (m + 1) * 4 + n * 4 + m * 4
+    del ptr, idx, data, x, y, handle
+    return {"rows": m, "cols": n, "nnz": nnz, "bytes": nbytes, "event_ms_median": round(med, 5),
+            "gbs": round(nbytes / (med * 1e-3) / 1e9, 2), "hbm_frac": round(nbytes / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def bench_bsr(dev, steps, warmup):
+    """BSR 65536 x 65536 (16 x 16 blocks, 40 per block row, seed 2) times
+    B 65536 x 128 on the matrix cores: bf16, f16 and fp8 e4m3 (per-block
+    absmax scales for A, one absmax scale for B).  Kernel time only (B
+    packed and A quantised beforehand)."""
+    from flagsparse_amd import _lib, bsr  # noqa: PLC0415
+
+    log("generating BSR 65536^2, 40 blocks per block row (seed 2)")
+    bp, bc, bd = synthetic.bsr_blocks()
+    B = np.random.default_rng(2).standard_normal((65536, 128))
+    nnzb = int(bc.size)
+    indptr = torch.from_numpy(bp).to(dev, torch.int32)
+    indices = torch.from_numpy(bc).to(dev, torch.int32)
+    blocks32 = torch.from_numpy(bd).to(dev, torch.float32)
+    Bt = synthetic.dense(B, dev, torch.float32)
+    C = torch.empty((65536, 128), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    flops = 2.0 * nnzb * 256 * 128
+    peaks = {"bf16": 2500.0, "f16": 2500.0, "fp8": 5000.0}  # dense MFMA TFLOP/s
+    out = {"nnzb": nnzb, "flops": flops}
+    lib = _lib.load()
+    for name in ("bf16", "f16", "fp8"):
+        if name == "fp8":
+            data, scales = bsr.quantize_bsr_fp8(blocks32)
+            Bp, absmax = bsr.pack_dense_operand(Bt, torch.uint8)
+            vcode, elem = _lib.FSP_F8E4M3, 1
+        else:
+            dt = torch.bfloat16 if name == "bf16" else torch.float16
+            data, scales = blocks32.to(dt), None
+            Bp, absmax = bsr.pack_dense_operand(Bt, dt)
+            vcode, elem = _lib.value_code(dt), 2
+
+        def step():
+            _lib.check(lib.fsp_bsr_spmm(vcode, _lib.FSP_I32, indptr.data_ptr(), indices.data_ptr(), data.data_ptr(),
+                                        None if scales is None else scales.data_ptr(), 4096, nnzb, Bp.data_ptr(),
+                                        None if absmax is None else absmax.data_ptr(), 65536, 128, C.data_ptr(), 128,
+                                        65536, stream.cuda_stream), "fsp_bsr_spmm")
+
+        _, ev = time_steps(step, steps, warmup, stream)
+        med = statistics.median(ev)
+        # algorithmic bytes: blocks (+ scales), block columns and row pointer,
+        # B once in the kernel's storage type, C (float32) once
+        nbytes = nnzb * 256 * elem + nnzb * 4 + 4097 * 4 + 65536 * 128 * elem + 65536 * 128 * 4
+        if name == "fp8":
+            nbytes += nnzb * 4
+        tflops = flops / (med * 1e-3) / 1e12
+        out[name] = {"event_ms_median": round(med, 5), "tflops": round(tflops, 2),
+                     "mfma_util": round(tflops / peaks[name], 4), "bytes": nbytes,
+                     "gbs": round(nbytes / (med * 1e-3) / 1e9, 2),
+                     "hbm_frac": round(nbytes / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        del data, Bp
+    return out
+
+
+def bench_batch(dev, steps, warmup):
+    """4096 square CSR matrices (rows log-uniform in [64, 8192], 4..64
+    nonzeros per row, seed 4) in one launch, HIP-graph replayed: SpMV and
+    SpMM (k = 32) with autotuned parameters, against the per-matrix launch
+    loop (4096 launches) of the same kernels."""
+    from flagsparse_amd.batch import CsrBatch  # noqa: PLC0415
+
+    log("generating the 4096-matrix batch (seed 4)")
+    ip, ix, v, ro, no = synthetic.csr_batch(seed=4, device=dev)
+    b = CsrBatch(v, ix, ip, ro, None, no)
+    del ip, ix, ro, no
+    torch.cuda.empty_cache()
+    rows, nnz = b.total_rows, b.nnz
+    x = torch.rand(b.total_cols, device=dev) * 2 - 1
+    y = torch.empty(rows, device=dev)
+    Bm = torch.randn(b.total_cols, 32, device=dev)
+    Cm = torch.empty(rows, 32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    out = {"matrices": b.n_mats, "rows": rows, "nnz": nnz}
+    log("tuning the batch kernels")
+    pv = b.params("spmv")
+    pm = b.params("spmm", 32)
+    out["tuned"] = {"spmv": pv.__dict__, "spmm": pm.__dict__}
+    gv = b.graph_spmv(x, y)
+    gm = b.graph_spmm(Bm, Cm)
+    bytes_v = nnz * 8 + (rows + 1) * 4 + b.total_cols * 4 + rows * 4
+    bytes_m = nnz * 8 + (rows + 1) * 4 + b.total_cols * 32 * 4 + rows * 32 * 4
+    for name, g, nbytes, loop in (("spmv", gv, bytes_v, lambda: b.spmv_per_matrix(x, out=y, params=pv)),
+                                  ("spmm_k32", gm, bytes_m, lambda: b.spmm_per_matrix(Bm, out=Cm, params=pm))):
+        _, ev = time_steps(g.replay, steps, warmup, stream)
+        med = statistics.median(ev)
+        loop()  # per-matrix plans are built before the loop is timed
+        wall_loop, ev_loop = time_steps(loop, max(3, steps // 10), 1, stream)
+        loop_ms = wall_loop / max(3, steps // 10)
+        out[name] = {"graph_event_ms_median": round(med, 5), "bytes": nbytes,
+                     "gbs": round(nbytes / (med * 1e-3) / 1e9, 2),
+                     "hbm_frac": round(nbytes / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "per_matrix_loop_ms": round(loop_ms, 4),
+                     "speedup_vs_loop": round(loop_ms / med, 2)}
+    del b, x, y, Bm, Cm, gv, gm
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(inputs, budget_s=20.0):
     """C oracle (one core) on the first rows of the same workload, scaled to
     GB/s with the same algorithmic-byte model; plus torch.sparse on all cores."""
@@ -220,6 +353,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="headline SpMM only (profiling runs)")
+    ap.add_argument("--with-copy", action="store_true",
+                    help="also run the 1 GiB copy (the counter calibration kernel) under --no-extras")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the hot path is a single-GPU kernel); got --gpus {args.gpus}",
@@ -275,7 +410,11 @@ def main():
             "ms_per_step": round(r["trans_ms"], 5),
         },
     }
+    if args.no_extras and args.with_copy:
+        line["hbm_copy_1gib"] = bench_copy(dev, args.steps)
     if not args.no_extras:
+        line["hbm_copy_1gib"] = bench_copy(dev, args.steps)
+        line["spmv_uniform_16m_rows_x16"] = bench_spmv_large(dev, args.steps, args.warmup)
         s = bench_spmv_uniform(dev, args.steps, args.warmup)
         line["spmv_uniform_16384x16"] = {"ms_per_step": round(s["ms"], 5),
                                          "event_ms_median": round(s["event_ms_median"], 5),
@@ -283,6 +422,8 @@ def main():
         a = bench_autograd_bf16_vs_fp32(dev, max(5, args.steps // 5), args.warmup)
         line["bf16_fwd_bwd_speedup_vs_fp32"] = round(a["speedup_bf16_vs_fp32"], 3)
         line["autograd_200k"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in a.items()}
+        line["bsr_65536_b40_n128"] = bench_bsr(dev, args.steps, args.warmup)
+        line["batch_4096"] = bench_batch(dev, args.steps, args.warmup)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         cb = cpu_baseline(r["inputs"])

@@ -252,3 +252,28 @@ def csr_batch(n_mats=4096, rows_lo=64, rows_hi=8192, nnz_lo=4, nnz_hi=64, seed=4
     del lo, hi, pick
     values = torch.rand(total, generator=g, dtype=torch.float32, device=dev) * 2.0 - 1.0
     return indptr, indices, values, row_offsets, nnz_offsets
+
+
+def uniform_rows_device(n_rows, n_cols, per_row, seed, device="cpu"):
+    """Large uniform-row CSR built with torch's seeded generator on `device`:
+    every row has `per_row` distinct ascending columns, one uniform column
+    from each of the integer strata [floor(j n / L), floor((j + 1) n / L));
+    values and x ~ U[-1, 1] (float32); int32 indices and row pointer.
+    Returns (indptr, indices, values, x) on `device`."""
+    dev = torch.device(device)
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    L = int(per_row)
+    if L > n_cols:
+        raise ValueError("per_row exceeds n_cols")
+    nnz = int(n_rows) * L
+    indptr = torch.arange(0, nnz + 1, L, dtype=torch.int64, device=dev).to(torch.int32)
+    j = torch.arange(L, dtype=torch.int64, device=dev)
+    lo = (j * n_cols) // L
+    width = ((j + 1) * n_cols) // L - lo
+    pick = torch.rand((int(n_rows), L), generator=g, dtype=torch.float64, device=dev)
+    indices = (lo + torch.floor(pick * width).to(torch.int64)).to(torch.int32).reshape(-1)
+    del pick
+    values = torch.rand(nnz, generator=g, dtype=torch.float32, device=dev) * 2.0 - 1.0
+    x = torch.rand(int(n_cols), generator=g, dtype=torch.float32, device=dev) * 2.0 - 1.0
+    return indptr, indices, values, x

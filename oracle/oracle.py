@@ -215,6 +215,56 @@ def fp8_tensor_quantize_f32(x):
     return codes, np.float32(scale)
 
 
+def mtx_to_csr_loop(lines):
+    """Matrix Market lines -> (values float64, indices, indptr, shape) by the
+    reference loader's own procedure (tests/test_spmv.py:116-189): one dict
+    per row, every entry (and its symmetric / skew mirror) added in file
+    order starting from 0.0, columns sorted per row.  Pure Python: small
+    files only."""
+    field, symmetry = "real", "general"
+    header = None
+    data_lines = []
+    for line in lines:
+        s = line.strip()
+        if s.startswith("%%MatrixMarket"):
+            t = s.split()
+            if len(t) >= 5:
+                field, symmetry = t[3].lower(), t[4].lower()
+            continue
+        if s.startswith("%"):
+            continue
+        if header is None and s:
+            p = s.split()
+            header = (int(p[0]), int(p[1]), int(p[2]) if len(p) > 2 else 0)
+            continue
+        if s:
+            data_lines.append(s)
+    if header is None:
+        raise ValueError("Cannot parse .mtx header")
+    n_rows, n_cols, nnz = header
+    if nnz == 0:
+        return np.zeros(0), np.zeros(0, np.int64), np.zeros(n_rows + 1, np.int64), (n_rows, n_cols)
+    sym = symmetry in ("symmetric", "hermitian")
+    skew = symmetry == "skew-symmetric"
+    maps = [dict() for _ in range(n_rows)]
+    for line in data_lines[:nnz]:
+        p = line.split()
+        r, c = int(p[0]) - 1, int(p[1]) - 1
+        v = 1.0 if field == "pattern" else float(p[2])
+        if 0 <= r < n_rows and 0 <= c < n_cols:
+            maps[r][c] = maps[r].get(c, 0.0) + v
+            if r != c and (sym or skew) and 0 <= c < n_rows and 0 <= r < n_cols:
+                maps[c][r] = maps[c].get(r, 0.0) + (-v if skew else v)
+    vals, cols, ptr = [], [], [0]
+    for m in maps:
+        for c in sorted(m):
+            cols.append(c)
+            vals.append(m[c])
+        ptr.append(len(cols))
+    return np.array(vals, dtype=np.float64), np.array(cols, dtype=np.int64), np.array(ptr, dtype=np.int64), \
+        (n_rows, n_cols)
+
+
 def max_relative_error(candidate, reference):
     """max|cand - ref| / max|ref| (spmm_csr.py:86-125)."""
     cand = np.asarray(candidate, dtype=np.float64)
