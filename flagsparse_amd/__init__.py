@@ -10,6 +10,9 @@ gfx950 in libflagsparse_hip.so; there is no other backend.
 Beyond the reference: the differentiable SpMM (``CsrMatrix``,
 ``spmm_csr_autograd``) and the BSR SpMM on the matrix cores
 (``csr_to_bsr``, ``quantize_bsr_fp8``, ``flagsparse_spmm_bsr``).
+Adjacent to the path: COO SpMV through a GPU COO -> CSR conversion
+(``flagsparse_spmv_coo_tocsr``) and Matrix Market ingestion
+(``load_mtx_to_csr_torch``).
 """
 
 from ._common import (
@@ -39,6 +42,8 @@ from .spmm_csr import (
     prepare_spmm_csr_route,
     resolve_spmm_csr_algorithm,
 )
+from .mtx import load_mtx_to_csr_torch
+from .spmv_coo import coo_to_csr_for_spmv, flagsparse_spmv_coo_tocsr, prepare_spmv_coo_tocsr
 from .spmv_csr import PreparedCsrSpmv, flagsparse_spmv_csr, prepare_spmv_csr
 
 __version__ = "0.1.0"
@@ -76,4 +81,8 @@ __all__ = [
     "csr_to_bsr",
     "quantize_bsr_fp8",
     "flagsparse_spmm_bsr",
+    "coo_to_csr_for_spmv",
+    "prepare_spmv_coo_tocsr",
+    "flagsparse_spmv_coo_tocsr",
+    "load_mtx_to_csr_torch",
 ]

@@ -204,6 +204,20 @@ int fsp_bsr_spmm(int32_t value_type, int32_t block_ptr_type, const void* block_p
                  int64_t n_block_rows, int64_t nnzb, const void* Bp_dev, const float* b_absmax_dev,
                  int64_t K, int64_t N, float* C_dev, int64_t ldc, int64_t M, void* stream);
 
+/* ---- COO -> CSR ------------------------------------------------------------
+ * Replaces coo_to_csr_for_spmv (spmv_csr.py:958-981: argsort of row * n_cols
+ * + col, bincount, cumsum).  Stable: duplicates keep their input order.
+ * Outputs: indptr [n_rows + 1], cols [nnz] (column of each sorted nonzero),
+ * perm [nnz] (input position of each sorted nonzero; gather the values with
+ * fsp_gather).  Row ids outside [0, n_rows) set bit 1 of *err_dev, columns
+ * outside [0, n_cols) bit 2 (the outputs are then meaningless but every
+ * write stays in bounds). */
+int fsp_coo_to_csr_workspace(int64_t n_rows, int64_t n_cols, int64_t nnz, int32_t perm_type, int64_t* bytes);
+int fsp_coo_to_csr(int32_t index_type, const void* row_dev, const void* col_dev, int64_t n_rows, int64_t n_cols,
+                   int64_t nnz, int32_t indptr_type, void* indptr_dev, int32_t out_index_type, void* cols_dev,
+                   int32_t perm_type, void* perm_dev, uint32_t* err_dev, void* ws_dev, int64_t ws_bytes,
+                   void* stream);
+
 /* ---- batches of CSR matrices --------------------------------------------
  * No counterpart in the reference (it runs one matrix per call).  A batch of
  * n_mats matrices is a packed offset table over concatenated arrays: matrix m

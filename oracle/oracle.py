@@ -215,6 +215,17 @@ def fp8_tensor_quantize_f32(x):
     return codes, np.float32(scale)
 
 
+def coo_to_csr(row, col, data, shape):
+    """(indptr, indices, data) of a COO matrix: nonzeros stably ordered by
+    (row, col), duplicates kept in input order (coo_to_csr_for_spmv,
+    spmv_csr.py:958-981, with a stable sort)."""
+    row, col = _i64(row), _i64(col)
+    order = np.lexsort((col, row))  # stable
+    indptr = np.zeros(int(shape[0]) + 1, dtype=np.int64)
+    np.cumsum(np.bincount(row, minlength=int(shape[0])), out=indptr[1:])
+    return indptr, col[order], np.asarray(data)[order]
+
+
 def mtx_to_csr_loop(lines):
     """Matrix Market lines -> (values float64, indices, indptr, shape) by the
     reference loader's own procedure (tests/test_spmv.py:116-189): one dict

@@ -235,3 +235,20 @@ def test_fp8_float32_quantizers():
     cb, sb = O.fp8_tensor_quantize_f32(B)
     assert sb == np.float32(np.abs(B.astype(np.float32)).max()) / np.float32(448)
     assert np.abs(O.e4m3_decode(cb)).max() == 448.0
+
+
+def test_coo_to_csr_matches_scipy():
+    rng = np.random.default_rng(12)
+    m, n, k = 30, 17, 400
+    row, col = rng.integers(0, m, k), rng.integers(0, n, k)
+    row[:50], col[:50] = row[-50:], col[-50:]  # duplicates
+    v = rng.standard_normal(k)
+    ip, ix, dv = O.coo_to_csr(row, col, v, (m, n))
+    mine = sp.csr_matrix((dv, ix, ip), shape=(m, n))
+    ref = sp.coo_matrix((v, (row, col)), shape=(m, n)).tocsr()  # sums duplicates
+    np.testing.assert_allclose(mine.toarray(), ref.toarray(), rtol=1e-13, atol=1e-13)
+    keys = ip.searchsorted(np.arange(ix.size), side="right") - 1  # row of each sorted nonzero
+    assert np.all(np.diff(keys * n + ix) >= 0)  # ordered by (row, col)
+    # stable: equal keys keep input order
+    order = np.lexsort((col, row))
+    np.testing.assert_array_equal(dv, v[order])
