@@ -11,7 +11,8 @@ Beyond the reference: the differentiable SpMM (``CsrMatrix``,
 ``spmm_csr_autograd``) and the BSR SpMM on the matrix cores
 (``csr_to_bsr``, ``quantize_bsr_fp8``, ``flagsparse_spmm_bsr``).
 Adjacent to the path: COO SpMV through a GPU COO -> CSR conversion
-(``flagsparse_spmv_coo_tocsr``) and Matrix Market ingestion
+(``flagsparse_spmv_coo_tocsr``), BSR SpMV (``flagsparse_spmv_bsr``) and
+Matrix Market ingestion
 (``load_mtx_to_csr_torch``).
 """
 
@@ -43,6 +44,7 @@ from .spmm_csr import (
     resolve_spmm_csr_algorithm,
 )
 from .mtx import load_mtx_to_csr_torch
+from .spmv_bsr import PreparedBsrSpmv, flagsparse_spmv_bsr, prepare_spmv_bsr
 from .spmv_coo import coo_to_csr_for_spmv, flagsparse_spmv_coo_tocsr, prepare_spmv_coo_tocsr
 from .spmv_csr import PreparedCsrSpmv, flagsparse_spmv_csr, prepare_spmv_csr
 
@@ -85,4 +87,7 @@ __all__ = [
     "prepare_spmv_coo_tocsr",
     "flagsparse_spmv_coo_tocsr",
     "load_mtx_to_csr_torch",
+    "PreparedBsrSpmv",
+    "prepare_spmv_bsr",
+    "flagsparse_spmv_bsr",
 ]

@@ -101,6 +101,8 @@ SIGNATURES = {
     "fsp_bsr_quantize_fp8": (_c_i32, [_c_i32, _c_vp, _c_i64, _c_vp, _c_vp, _c_vp]),
     "fsp_bsr_spmm": (_c_i32, [_c_i32, _c_i32, _c_vp, _c_vp, _c_vp, _c_vp, _c_i64, _c_i64, _c_vp, _c_vp, _c_i64,
                               _c_i64, _c_vp, _c_i64, _c_i64, _c_vp]),
+    "fsp_bsr_spmv": (_c_i32, [_c_i32, _c_i32, _c_i32, _c_vp, _c_vp, _c_i32, _c_vp, _c_vp, _c_i32, _c_vp, _c_vp, _c_i64,
+                              _c_vp]),
     "fsp_coo_to_csr_workspace": (_c_i32, [_c_i64, _c_i64, _c_i64, _c_i32, _c_vp]),
     "fsp_coo_to_csr": (_c_i32, [_c_i32, _c_vp, _c_vp, _c_i64, _c_i64, _c_i64, _c_i32, _c_vp, _c_i32, _c_vp, _c_i32,
                                 _c_vp, _c_vp, _c_vp, _c_i64, _c_vp]),

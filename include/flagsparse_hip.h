@@ -204,6 +204,20 @@ int fsp_bsr_spmm(int32_t value_type, int32_t block_ptr_type, const void* block_p
                  int64_t n_block_rows, int64_t nnzb, const void* Bp_dev, const float* b_absmax_dev,
                  int64_t K, int64_t N, float* C_dev, int64_t ldc, int64_t M, void* stream);
 
+/* ---- BSR SpMV ----------------------------------------------------------------
+ * y (padded: n_out_blocks * block_dim) = A x over a BSR matrix with square
+ * blocks of block_dim in [1, 32], data [nnzb][block_dim][block_dim]
+ * row-major.  op non: ptr = block row pointer, bidx = block columns,
+ * perm = NULL, x padded to n_block_cols * block_dim.  op trans / conj (real
+ * values): ptr / bidx / perm = block column pointer, block rows and block
+ * ids of the block CSC (fsp_csr_to_csc over the block structure), x padded
+ * to n_block_rows * block_dim.  Deterministic and atomic-free.  Replaces
+ * _spmv_bsr_non_real_kernel / _spmv_bsr_trans_real_kernel and the
+ * blockrow_reduce kernels (spmv_bsr.py:183-221, 275-318, 376-405, 603-757). */
+int fsp_bsr_spmv(int32_t value_type, int32_t ptr_type, int32_t index_type, const void* ptr_dev, const void* bidx_dev,
+                 int32_t perm_type, const void* perm_dev, const void* data_dev, int32_t block_dim, const void* x_dev,
+                 void* y_dev, int64_t n_out_blocks, void* stream);
+
 /* ---- COO -> CSR ------------------------------------------------------------
  * Replaces coo_to_csr_for_spmv (spmv_csr.py:958-981: argsort of row * n_cols
  * + col, bincount, cumsum).  Stable: duplicates keep their input order.
