@@ -162,6 +162,23 @@ def bench_autograd_bf16_vs_fp32(dev, steps, warmup):
     return out
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "spmm_bins_kernel_power_law_262144_n64_pmc.md")
+
+
+def measured_traffic(path=TRAFFIC_SUMMARY):
+    """HBM bytes per launch of the headline SpMM from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy
+    kernel's calibration factor); None when the summary is absent."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1].strip()))
+    except (OSError, ValueError):
+        return None
+    return None
+
+
 def bench_copy(dev, steps):
     """Measured HBM copy rate (1 GiB device-to-device, read + write bytes);
     its kernel also calibrates the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
@@ -400,7 +417,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(),
             "kernel": "spmm_bins_kernel (+ spmm_long_reduce_kernel inside the event window)",
             "kernel_ms": round(kern_ms, 5),
         },

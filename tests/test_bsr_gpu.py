@@ -10,7 +10,8 @@
 * MFMA BSR SpMM: bf16 / f16 against the FP64 oracle on the rounded inputs
   (float32 accumulation, rel 1e-5) and on the unrounded inputs (the bf16 /
   f16 storage bound); fp8 against the oracle on the dequantised operands
-  (rel 1e-5) with the error against unquantised FP64 reported.
+  (rel 1e-4, see FP8_VS_DEQUANT_TOL) with the error against unquantised FP64
+  reported.
 """
 
 import numpy as np
@@ -27,6 +28,12 @@ from ._helpers import REL_TOL, np_dtype_round, rel_err
 pytestmark = pytest.mark.gpu
 
 BS = 16
+# The fp8 MFMA (v_mfma_f32_16x16x32_fp8_fp8) handles e4m3 subnormals exactly
+# but sums its 32 products below float32 precision: on random full-range
+# codes |D - exact| reaches ~6e-5 of max|D| (measured on MI355X).  Against the
+# oracle on the dequantised operands the fp8 path is therefore held to 1e-4,
+# the bf16 / f16 paths (exact products, float32 sums) to 1e-5.
+FP8_VS_DEQUANT_TOL = 1e-4
 
 
 def _dev_bsr(bp, bc, bd, device, dtype, indptr_dtype=torch.int32):
@@ -176,7 +183,7 @@ def test_bsr_spmm_fp8(device, case):
     b_deq = O.e4m3_decode(b_codes) * np.float64(b_scale)
     want = O.bsr_spmm(bp, bc, a_deq, _padded_b(b_deq, nbc))[:n_rows]
     err = rel_err(C, want)
-    assert err <= 1e-5, f"vs oracle on the dequantised operands: {err:.3e}"
+    assert err <= FP8_VS_DEQUANT_TOL, f"vs oracle on the dequantised operands: {err:.3e}"
     err64 = rel_err(C, O.bsr_spmm(bp, bc, bd, _padded_b(B, nbc))[:n_rows])
     print(f"[fp8] rel err vs unquantised FP64: {err64:.3e}")
     assert err64 <= 0.1
@@ -248,7 +255,7 @@ def test_full_bsr_spmm(device, full_bsr, dtype):
     # first 64 block rows (1024 rows) against the oracle
     sample = C[:1024]
     want = O.bsr_spmm(bp[:65], bc[: bp[64]], a_use, b_use)
-    assert rel_err(sample, want) <= 1e-5
+    assert rel_err(sample, want) <= (FP8_VS_DEQUANT_TOL if dtype == "fp8" else 1e-5)
     exact = O.bsr_spmm(bp[:65], bc[: bp[64]], bd[: bp[64]], B)
     err64 = rel_err(sample, exact)
     print(f"[{dtype}] rel err vs FP64 (1024 rows): {err64:.3e}")

@@ -18,7 +18,7 @@ def mtx_file(tmp_path_factory):
     c = rng.integers(1, n + 1, 40000)
     v = rng.standard_normal(40000)
     lines = ["%%MatrixMarket matrix coordinate real symmetric", f"{n} {n} {r.size}"]
-    lines += [f"{max(a, b)} {min(a, b)} {x!r}" for a, b, x in zip(r, c, v)]
+    lines += [f"{max(a, b)} {min(a, b)} {float(x)!r}" for a, b, x in zip(r, c, v)]
     path = tmp_path_factory.mktemp("mtx") / "sym.mtx"
     path.write_text("\n".join(lines) + "\n")
     return str(path)
