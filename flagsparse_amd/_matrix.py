@@ -54,9 +54,10 @@ class CsrHandle:
         return p
 
     # --------------------------------------------------------------- kernels
-    def spmm(self, data, B, transpose=False, out=None, lanes_n=None, config=None, max_blocks=0):
+    def spmm(self, data, B, transpose=False, out=None, lanes_n=None, config=None, max_blocks=0, xcd_groups=0):
         """C = A B (transpose=False) or A^T B (transpose=True); B row-major.
-        max_blocks > 0: persistent launch of at most that many workgroups."""
+        max_blocks > 0: persistent launch of at most that many workgroups;
+        xcd_groups > 1: XCD-region tile mapping (see _plan.CsrPlan.struct)."""
         n_out = self.n_cols if transpose else self.n_rows
         n_dense = int(B.shape[1])
         device = B.device
@@ -76,7 +77,7 @@ class CsrHandle:
             plan = self.plan(cfg)
             ptr_t, idx_t, perm_t = self.indptr, self.indices, None
         partials = plan.partials(n_dense, _acc_dtype(data.dtype), device)
-        st = plan.struct(partials, max_blocks)
+        st = plan.struct(partials, max_blocks, xcd_groups)
         _lib.check(
             _lib.load().fsp_spmm_csr(
                 _lib.value_code(data.dtype),
@@ -103,7 +104,7 @@ class CsrHandle:
         )
         return C
 
-    def spmv(self, data, x, transpose=False, out=None, config=None, max_blocks=0):
+    def spmv(self, data, x, transpose=False, out=None, config=None, max_blocks=0, xcd_groups=0):
         n_out = self.n_cols if transpose else self.n_rows
         y = out if out is not None else torch.empty(n_out, dtype=data.dtype, device=x.device)
         if n_out == 0:
@@ -118,7 +119,7 @@ class CsrHandle:
             ptr_t, idx_t, perm_t = self.indptr, self.indices, None
         acc = torch.float64 if data.dtype in (torch.float32, torch.float64) else torch.float32
         partials = plan.partials(1, acc, x.device)
-        st = plan.struct(partials, max_blocks)
+        st = plan.struct(partials, max_blocks, xcd_groups)
         _lib.check(
             _lib.load().fsp_spmv_csr(
                 _lib.value_code(data.dtype),
@@ -139,7 +140,8 @@ class CsrHandle:
         )
         return y
 
-    def sddmm(self, X, Y, alpha=1.0, beta=0.0, data_in=None, out=None, lanes_k=None, config=None, max_blocks=0):
+    def sddmm(self, X, Y, alpha=1.0, beta=0.0, data_in=None, out=None, lanes_k=None, config=None, max_blocks=0,
+              xcd_groups=0):
         """out[p] = alpha <X[row p], Y[col p]> + beta data_in[p] on this pattern."""
         k = int(X.shape[1])
         device = X.device
@@ -152,10 +154,10 @@ class CsrHandle:
             else:
                 out.copy_(data_in * beta)
             return out
-        G = int(lanes_k) if lanes_k else _plan.lanes_for_dense(k, X.element_size())
+        G = int(lanes_k) if lanes_k else _plan.sddmm_lanes(k, X.element_size())
         cfg = config or _plan.sddmm_config(G)
         plan = self.plan(cfg)
-        st = plan.struct(None, max_blocks)
+        st = plan.struct(None, max_blocks, xcd_groups)
         _lib.check(
             _lib.load().fsp_sddmm_csr(
                 _lib.value_code(X.dtype),

@@ -12,7 +12,9 @@ The tuned parameters are the ones the kernels take from their plan:
                      ``seg_len`` nonzeros (= long_threshold);
 * ``blocks_per_cu``  0 = one workgroup per row tile, else a persistent grid of
                      blocks_per_cu workgroups per compute unit, each walking a
-                     contiguous run of rows.
+                     contiguous run of rows;
+* ``xcd_groups``     8 = cut each row bin into 8 contiguous regions, one per
+                     XCD, so neighbouring rows share an L2 (0 = off).
 
 A shape class is (kernel, value / index dtypes, dense width, rounded log2 of
 the mean and maximum row length and of the row count, compute units).  The
@@ -47,6 +49,7 @@ class TuneParams:
     lanes_n: int = 0  # 0 = lanes_for_dense(n_dense)
     long_threshold: int = 0  # 0 = the kernel family's default
     blocks_per_cu: int = 0
+    xcd_groups: int = 0
 
     def spmv_config(self):
         lt = self.long_threshold or _plan.SPMV_LONG_THRESHOLD
@@ -93,7 +96,8 @@ def candidates(kind, n_dense=1, elem_bytes=4):
     if kind == "spmv":
         for unroll in (2, 4, 8):
             for bpc in (0, 4, 8):
-                out.append(TuneParams(unroll=unroll, blocks_per_cu=bpc))
+                for xg in (0, 8):
+                    out.append(TuneParams(unroll=unroll, blocks_per_cu=bpc, xcd_groups=xg))
         return out
     g = _plan.lanes_for_dense(n_dense, elem_bytes)
     lanes = [g] + ([g // 2] if g >= 8 else [])
@@ -101,7 +105,9 @@ def candidates(kind, n_dense=1, elem_bytes=4):
         for unroll in (4, 8):
             for scale in (1, 2, 4):
                 for bpc in (0, 8):
-                    out.append(TuneParams(unroll=unroll, slot_scale=scale, lanes_n=ln, blocks_per_cu=bpc))
+                    for xg in (0, 8):
+                        out.append(TuneParams(unroll=unroll, slot_scale=scale, lanes_n=ln, blocks_per_cu=bpc,
+                                              xcd_groups=xg))
     return out
 
 

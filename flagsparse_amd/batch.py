@@ -160,7 +160,8 @@ class CsrBatch:
                 self.handle.plan(p.spmv_config())
 
             def run(p):
-                self.handle.spmv(data, x, out=y, config=p.spmv_config(), max_blocks=p.max_blocks(self.device))
+                self.handle.spmv(data, x, out=y, config=p.spmv_config(), max_blocks=p.max_blocks(self.device),
+                                 xcd_groups=p.xcd_groups)
         else:
             B = torch.ones((self.total_cols, n_dense), dtype=dtype, device=self.device)
             C = torch.empty((self.total_rows, n_dense), dtype=dtype, device=self.device)
@@ -170,7 +171,7 @@ class CsrBatch:
             def run(p):
                 self.handle.spmm(data, B, out=C, lanes_n=self._lanes(p, n_dense, elem),
                                  config=p.spmm_config(self._lanes(p, n_dense, elem)),
-                                 max_blocks=p.max_blocks(self.device))
+                                 max_blocks=p.max_blocks(self.device), xcd_groups=p.xcd_groups)
         best, self.tune_results = autotune.tune(key, cands, run)
         return best
 
@@ -193,7 +194,7 @@ class CsrBatch:
             raise ValueError("x must be 1D with the values' dtype")
         p = params or self.params("spmv", 1, data.dtype)
         return self.handle.spmv(data, x.contiguous(), out=out, config=p.spmv_config(),
-                                max_blocks=p.max_blocks(self.device))
+                                max_blocks=p.max_blocks(self.device), xcd_groups=p.xcd_groups)
 
     def spmm(self, B, out=None, data=None, params=None):
         """C (packed by rows) = blockdiag(A_m) B (packed by rows): one launch."""
@@ -205,7 +206,7 @@ class CsrBatch:
         p = params or self.params("spmm", n_dense, data.dtype)
         lanes = self._lanes(p, n_dense, data.element_size())
         return self.handle.spmm(data, B.contiguous(), out=out, lanes_n=lanes, config=p.spmm_config(lanes),
-                                max_blocks=p.max_blocks(self.device))
+                                max_blocks=p.max_blocks(self.device), xcd_groups=p.xcd_groups)
 
     # ----------------------------------------------------- graph capture
     def _capture(self, fn):

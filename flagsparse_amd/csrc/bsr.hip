@@ -6,15 +6,14 @@
 // a block row, indptr over block rows.
 //
 // SpMM C[M, N] (f32) = A_bsr B: one workgroup (4 waves) per block row and
-// 128-column chunk.  B is first packed into the MFMA operand layout
-// Bp[k / 8][n][k % 8] (8 consecutive B rows of one column = one 16-byte
-// operand for bf16/f16, 8 bytes for fp8), so every operand is one contiguous
-// load.  bf16 / f16: one v_mfma_f32_16x16x32 per block pair and 16-column
-// tile (lanes 0-31 carry the first block, lanes 32-63 the second).  FP8 (OCP
-// e4m3): two 16x16x32 fp8 MFMAs per pair, each with the other block's half of
-// K zeroed, so each block's product gets its own absmax scale before it is
-// accumulated.  The 4 waves split the block list; their 16 x 128 partial tiles
-// are added in LDS in wave order (deterministic) and written once.
+// 128-column chunk; each wave owns 32 of the columns.  B is first packed into
+// the MFMA operand layout Bp[k / 8][n][k % 8] (8 consecutive B rows of one
+// column = one 16-byte operand for bf16/f16, 8 bytes for fp8), so every
+// operand is one contiguous load.  bf16 / f16: one v_mfma_f32_16x16x32 per
+// block pair and 16-column tile (lanes 0-31 carry the first block, lanes
+// 32-63 the second).  FP8 (OCP e4m3): two 16x16x32 fp8 MFMAs per pair, each
+// with the other block's half of K zeroed, so each block's product gets its
+// own absmax scale before it is accumulated.
 #include <algorithm>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
@@ -289,92 +288,112 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void bsr_quantize_fp8_kernel(const 
 
 // ------------------------------------------------------------ BSR SpMM
 // KIND 0 = bf16, 1 = f16, 2 = fp8 (codes + per-block scales).
+// One workgroup per (block row, 128 columns); wave w owns the two 16-column
+// tiles 2w, 2w+1 and walks every block pair of the block row (lanes 0-31 carry
+// block j, 32-63 block j+1), so each C element is summed by one lane in block
+// order (deterministic, no cross-wave reduction, no LDS).  The next pair's A
+// and B fragments are loaded while the current pair's MFMAs run.
+constexpr int kTilesPerWave = kNt / FSP_WAVES;  // 2
+
 template <int KIND, typename BP>
 __global__ __launch_bounds__(FSP_WG_THREADS) void bsr_spmm_kernel(
     const BP* __restrict__ block_ptr, const int32_t* __restrict__ block_col, const void* __restrict__ block_data,
     const float* __restrict__ block_scales, const void* __restrict__ Bp, const float* __restrict__ b_absmax,
     int64_t nbc, int64_t Npad, int64_t N, float* __restrict__ C, int64_t ldc, int64_t M) {
-  __shared__ f32x4 red[FSP_WAVES][kNt][64];
+  using V = typename std::conditional<KIND == 0, bf16x8, f16x8>::type;
+  using E = typename std::conditional<KIND == 0, __bf16, _Float16>::type;
+  using AF = typename std::conditional<KIND == 2, uint64_t, V>::type;  // one lane's A (or B) fragment
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int64_t br = blockIdx.x;
-  const int64_t nbase = static_cast<int64_t>(blockIdx.y) * (kNt * 16);
+  const int64_t nbase = static_cast<int64_t>(blockIdx.y) * (kNt * 16) + wave * (kTilesPerWave * 16);
   const int64_t beg = static_cast<int64_t>(block_ptr[br]);
   const int64_t end = static_cast<int64_t>(block_ptr[br + 1]);
-  const int half = lane >> 5;              // which block of the pair this lane carries
-  const int r = lane & 15;                 // A row / B column inside the tile
-  const int kh = (lane >> 4) & 1;          // which 8-wide half of the block's K
-  f32x4 acc[kNt];
-#pragma unroll
-  for (int t = 0; t < kNt; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int half = lane >> 5;      // which block of the pair this lane carries
+  const int r = lane & 15;         // A row / B column inside the tile
+  const int kh = (lane >> 4) & 1;  // which 8-wide half of the block's K
+  const int64_t last = end > beg ? end - 1 : beg;  // a stored block (clamped loads)
 
-  for (int64_t j = beg + 2 * wave; j < end; j += 2 * FSP_WAVES) {
+  struct frag {
+    AF a;
+    AF b[kTilesPerWave];
+    bool valid;
+  };
+  // Fragments of pair j (unconditional, clamped loads; invalid lanes zeroed).
+  auto load = [&](int64_t j) -> frag {
+    frag f;
     const int64_t blk = j + half;
-    const int64_t bc = blk < end ? blk : j;  // a stored block for the address
+    const int64_t bc = blk < end ? blk : last;
     const int64_t col = static_cast<int64_t>(block_col[bc]);
-    // a block column outside [0, nbc) contributes nothing and reads nothing
-    const bool valid = blk < end && col >= 0 && col < nbc;
-    const int64_t k8 = (valid ? col : 0) * 2 + kh;
+    f.valid = blk < end && col >= 0 && col < nbc;  // an out-of-range column reads nothing useful
+    const int64_t k8 = (f.valid ? col : 0) * 2 + kh;
     if constexpr (KIND == 2) {
       const uint8_t* A8 = static_cast<const uint8_t*>(block_data);
       const uint64_t* B8 = static_cast<const uint64_t*>(Bp);
-      const uint64_t a_raw = *reinterpret_cast<const uint64_t*>(A8 + bc * 256 + r * 16 + kh * 8);
-      const uint64_t a = valid ? a_raw : 0ull;
-      const long a_lo = half == 0 ? static_cast<long>(a) : 0l;
-      const long a_hi = half == 1 ? static_cast<long>(a) : 0l;
-      const float s0 = block_scales[j];
-      const float s1 = (j + 1 < end) ? block_scales[j + 1] : 0.f;
+      const uint64_t a = *reinterpret_cast<const uint64_t*>(A8 + bc * 256 + r * 16 + kh * 8);
+      f.a = f.valid ? a : 0ull;
 #pragma unroll
-      for (int t = 0; t < kNt; ++t) {
-        const int64_t n = nbase + t * 16 + r;
-        const uint64_t b_raw = B8[k8 * Npad + n];
-        const long b = valid ? static_cast<long>(b_raw) : 0l;
-        const f32x4 z = {0.f, 0.f, 0.f, 0.f};
-        const f32x4 p0 = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a_lo, b, z, 0, 0, 0);
-        const f32x4 p1 = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a_hi, b, z, 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) acc[t][i] += s0 * p0[i] + s1 * p1[i];
+      for (int t = 0; t < kTilesPerWave; ++t) {
+        const uint64_t b = B8[k8 * Npad + nbase + t * 16 + r];
+        f.b[t] = f.valid ? b : 0ull;
       }
     } else {
-      using V = typename std::conditional<KIND == 0, bf16x8, f16x8>::type;
-      using E = typename std::conditional<KIND == 0, __bf16, _Float16>::type;
       const E* Ae = static_cast<const E*>(block_data);
       const E* Be = static_cast<const E*>(Bp);
-      V a = *reinterpret_cast<const V*>(Ae + bc * 256 + r * 16 + kh * 8);
-      if (!valid) a = V{};
+      const V a = *reinterpret_cast<const V*>(Ae + bc * 256 + r * 16 + kh * 8);
+      f.a = f.valid ? a : V{};
 #pragma unroll
-      for (int t = 0; t < kNt; ++t) {
-        const int64_t n = nbase + t * 16 + r;
-        V b = *reinterpret_cast<const V*>(Be + (k8 * Npad + n) * 8);
-        if (!valid) b = V{};
-        if constexpr (KIND == 0) {
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
-        } else {
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[t], 0, 0, 0);
-        }
+      for (int t = 0; t < kTilesPerWave; ++t) {
+        const V b = *reinterpret_cast<const V*>(Be + (k8 * Npad + nbase + t * 16 + r) * 8);
+        f.b[t] = f.valid ? b : V{};
       }
     }
+    return f;
+  };
+
+  f32x4 acc[kTilesPerWave];
+#pragma unroll
+  for (int t = 0; t < kTilesPerWave; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  if (beg < end) {
+    frag cur = load(beg);
+    for (int64_t j = beg; j < end; j += 2) {  // wave-uniform
+      const frag nxt = load(j + 2 < end ? j + 2 : j);  // prefetch (a repeat of j past the end)
+      if constexpr (KIND == 2) {
+        const float s0 = block_scales[j];
+        const float s1 = (j + 1 < end) ? block_scales[j + 1] : 0.f;
+        const long a_lo = half == 0 ? static_cast<long>(cur.a) : 0l;
+        const long a_hi = half == 1 ? static_cast<long>(cur.a) : 0l;
+#pragma unroll
+        for (int t = 0; t < kTilesPerWave; ++t) {
+          const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+          const f32x4 p0 = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a_lo, static_cast<long>(cur.b[t]), z, 0, 0, 0);
+          const f32x4 p1 = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a_hi, static_cast<long>(cur.b[t]), z, 0, 0, 0);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) acc[t][i] += s0 * p0[i] + s1 * p1[i];
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < kTilesPerWave; ++t) {
+          if constexpr (KIND == 0) {
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur.a, cur.b[t], acc[t], 0, 0, 0);
+          } else {
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(cur.a, cur.b[t], acc[t], 0, 0, 0);
+          }
+        }
+      }
+      cur = nxt;
+    }
   }
-#pragma unroll
-  for (int t = 0; t < kNt; ++t) red[wave][t][lane] = acc[t];
-  __syncthreads();
   const float bscale = KIND == 2 ? (b_absmax[0] > 0.f ? b_absmax[0] / kE4m3Max : 1.f) : 1.f;
-  // 16 rows x 128 columns per workgroup, 8 elements per thread: element e =
-  // tid + 256 i -> column n = e % 16 of tile t = e / 256, row (e / 16) % 16.
+  // D layout: lane l holds rows (l / 16) * 4 + i, column l % 16 of a tile.
 #pragma unroll
-  for (int i = 0; i < kNt; ++i) {
-    const int e = threadIdx.x + FSP_WG_THREADS * i;
-    const int t = e >> 8;
-    const int row = (e >> 4) & 15;
-    const int n = e & 15;
-    const int src_lane = (row >> 2) * 16 + n;  // lane holding D[row][n] in the MFMA output layout
-    const int jj = row & 3;
-    float v = 0.f;
+  for (int t = 0; t < kTilesPerWave; ++t) {
+    const int64_t gcol = nbase + t * 16 + (lane & 15);
 #pragma unroll
-    for (int w = 0; w < FSP_WAVES; ++w) v += red[w][t][src_lane][jj];
-    const int64_t grow = br * kBs + row;
-    const int64_t gcol = nbase + t * 16 + n;
-    if (grow < M && gcol < N) C[grow * ldc + gcol] = v * bscale;
+    for (int i = 0; i < 4; ++i) {
+      const int64_t grow = br * kBs + (lane >> 4) * 4 + i;
+      if (grow < M && gcol < N) C[grow * ldc + gcol] = acc[t][i] * bscale;
+    }
   }
 }
 

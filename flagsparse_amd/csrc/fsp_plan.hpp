@@ -6,8 +6,15 @@
 // Inside a bin, workgroup j processes the contiguous row tiles
 // [j * chunk, (j + 1) * chunk).  chunk is 1 unless the plan caps the grid
 // (max_blocks, a persistent launch sized to the chip): then each workgroup
-// walks a contiguous run of rows, which keeps the dense rows a run of the
-// matrix gathers (one matrix of a packed batch, say) in one XCD's L2.
+// walks a contiguous run of rows.
+//
+// With plan.xcd_groups = X > 1 the tiles of a bin are instead cut into X
+// contiguous regions and the workgroups b with b % X == x stride through
+// region x together (tile = start_x + b / X, + the group's size, ...).
+// Workgroups are dispatched round-robin over the XCDs (8 on MI355X), so with
+// X = 8 the workgroups resident on one XCD at any moment work on neighbouring
+// rows: the dense rows those rows gather (one matrix of a packed batch, say)
+// stay in that XCD's L2 instead of being pulled into all eight.
 #pragma once
 
 #include "fsp_device.hpp"
@@ -23,7 +30,8 @@ struct launch_bin {
   int64_t count;         // rows (or segments) in the bin
   int64_t block_begin;   // first workgroup of this bin
   int64_t nblocks;       // workgroups of this bin
-  int64_t chunk;         // consecutive tiles per workgroup
+  int64_t chunk;         // consecutive tiles per workgroup (xcds <= 1)
+  int64_t xcds;          // > 1: XCD-region mapping over this many groups
   int32_t lanes_log2;    // SpMV: lanes per row; SpMM/SDDMM: nonzero slots per row
   int32_t is_segment;    // 1 = items are long-row segments
 };
@@ -36,6 +44,29 @@ struct launch_table {
   const int64_t* seg_begin;
   const int64_t* seg_end;
 };
+
+// Tiles of one workgroup: first, first + step, ... while < end.
+struct tile_range {
+  int64_t first, step, end;
+};
+
+__device__ __forceinline__ tile_range bin_tiles(const launch_bin& lb, int64_t per_block) {
+  const int64_t n_tiles = (lb.count + per_block - 1) / per_block;
+  const int64_t b = static_cast<int64_t>(blockIdx.x) - lb.block_begin;
+  tile_range r;
+  if (lb.xcds <= 1) {
+    r.first = b * lb.chunk;
+    r.step = 1;
+    r.end = r.first + lb.chunk < n_tiles ? r.first + lb.chunk : n_tiles;
+  } else {
+    const int64_t X = lb.xcds;
+    const int64_t x = b % X;
+    r.first = n_tiles * x / X + b / X;
+    r.step = (lb.nblocks - x + X - 1) / X;  // workgroups of group x (>= 1: x < X <= nblocks)
+    r.end = n_tiles * (x + 1) / X;
+  }
+  return r;
+}
 
 // Work item = (row, first nonzero, one past the last nonzero, output slot).
 struct work_item {
@@ -123,8 +154,16 @@ static int64_t build_launch_table(const fsp_csr_plan* plan, int seg_lanes_log2, 
       if (nb < 1) nb = 1;
     }
     if (nb > kMaxBlocksPerBin) nb = kMaxBlocksPerBin;
-    lb.chunk = ceil_div(tiles[i], nb);
-    lb.nblocks = ceil_div(tiles[i], lb.chunk);  // no workgroup without tiles
+    const int64_t groups = plan->xcd_groups < nb ? plan->xcd_groups : nb;
+    if (groups > 1) {  // region mapping needs two or more workgroups
+      lb.chunk = 1;
+      lb.nblocks = nb;
+      lb.xcds = groups;
+    } else {
+      lb.chunk = ceil_div(tiles[i], nb);
+      lb.nblocks = ceil_div(tiles[i], lb.chunk);  // no workgroup without tiles
+      lb.xcds = 1;
+    }
     lb.block_begin = blocks;
     blocks += lb.nblocks;
   }
@@ -151,6 +190,9 @@ static inline int check_plan(const fsp_csr_plan* plan, int64_t n_rows) {
                 (long long)total, (long long)plan->n_long_rows, (long long)n_rows);
   }
   if (plan->n_segments < 0 || plan->n_long_rows < 0) return fail(FSP_ERR_ARG, "plan: negative segment counts");
+  if (plan->xcd_groups < 0 || plan->xcd_groups > 64) {
+    return fail(FSP_ERR_ARG, "plan: xcd_groups %lld outside [0, 64]", (long long)plan->xcd_groups);
+  }
   if (plan->max_blocks < 0 || plan->max_blocks > kMaxBlocksPerBin) {
     return fail(FSP_ERR_ARG, "plan: max_blocks %lld outside [0, %lld]", (long long)plan->max_blocks,
                 (long long)kMaxBlocksPerBin);

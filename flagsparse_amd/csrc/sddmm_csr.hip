@@ -59,9 +59,9 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void sddmm_bins_kernel(
   const int sub = lane >> (g_log2 + s_log2);
   const int64_t per_wave = 64 >> (g_log2 + s_log2);
   const int64_t per_block = per_wave * FSP_WAVES;
-  // this workgroup's contiguous run of tiles: items [i_begin, i_end)
-  const int64_t i_begin = (static_cast<int64_t>(blockIdx.x) - lb.block_begin) * lb.chunk * per_block;
-  const int64_t i_end = i_begin + lb.chunk * per_block < lb.count ? i_begin + lb.chunk * per_block : lb.count;
+  // this workgroup's tiles: first, first + step, ... < end
+  const tile_range tr = bin_tiles(lb, per_block);
+  const int64_t rr_step = tr.step * per_block;
   const O kk0 = static_cast<O>(gl) * VEC;
   const bool k_ok = static_cast<int64_t>(kk0) < k;
   const O kk_ld = k_ok ? kk0 : O(0);  // lanes past K read column 0 (masked)
@@ -121,15 +121,15 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void sddmm_bins_kernel(
   };
 
   if constexpr (ONE_CHUNK) {
-    int64_t rr = i_begin + wave * per_wave + sub;
+    int64_t rr = tr.first * per_block + wave * per_wave + sub;
     O row;
     P beg, end;
     item(rr, row, beg, end);
     V xv = *addr<OFF32>::at(reinterpret_cast<const V*>(X), (row * ldx_o + kk_ld) * static_cast<O>(sizeof(T)));
     IT c[UNROLL];
     trip(beg + slot, end, c);
-    for (int64_t base = i_begin; base < i_end; base += per_block) {  // wave-uniform
-      const int64_t rr_next = rr + per_block;
+    for (int64_t tile = tr.first; tile < tr.end; tile += tr.step) {  // wave-uniform
+      const int64_t rr_next = rr + rr_step;
       O row_n;
       P beg_n, end_n;
       item(rr_next, row_n, beg_n, end_n);
@@ -173,8 +173,8 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void sddmm_bins_kernel(
   } else {
     // K spans several passes of the G lanes: the X slice is re-read per pass.
     const O kstep = static_cast<O>(G) * VEC;
-    for (int64_t base = i_begin; base < i_end; base += per_block) {
-      const int64_t rr = base + wave * per_wave + sub;
+    for (int64_t tile = tr.first; tile < tr.end; tile += tr.step) {
+      const int64_t rr = tile * per_block + wave * per_wave + sub;
       O row;
       P beg, end;
       item(rr, row, beg, end);

@@ -46,9 +46,9 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
   const int64_t per_block = per_wave * FSP_WAVES;
   const O col0 = static_cast<O>((static_cast<int64_t>(blockIdx.y) * G + gl) * VEC);
   const bool col_ok = static_cast<int64_t>(col0) < n_dense;
-  // this workgroup's contiguous run of tiles: items [i_begin, i_end)
-  const int64_t i_begin = (static_cast<int64_t>(blockIdx.x) - lb.block_begin) * lb.chunk * per_block;
-  const int64_t i_end = i_begin + lb.chunk * per_block < lb.count ? i_begin + lb.chunk * per_block : lb.count;
+  // this workgroup's tiles: first, first + step, ... < end
+  const tile_range tr = bin_tiles(lb, per_block);
+  const int64_t rr_step = tr.step * per_block;
   const O ldb_o = static_cast<O>(ldb);
   const O ldc_o = static_cast<O>(ldc);
   const O nd_o = static_cast<O>(n_dense);
@@ -79,33 +79,34 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
     beg = live ? b : P(0);
     end = live ? e : P(0);
   };
-  // Column ids and values of nonzeros p, p+S, ... (zero past `end`).
-  auto trip = [&](P p, P end, IT (&c)[UNROLL], A (&a)[UNROLL]) {
+  // Column ids and values of nonzeros p, p+S, ...; past `end` the loads
+  // repeat the item's last nonzero (a valid column of the same row).  They
+  // stay raw: masking or converting them here would make the compiler wait
+  // for every load in flight (the B rows included) right after the prefetch,
+  // so the mask and the conversion are applied where they are consumed.
+  auto trip = [&](P p, P end, IT (&c)[UNROLL], T (&a)[UNROLL]) {
     const P last = end > 0 ? end - 1 : P(0);
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
       const P q = p + static_cast<P>(u) * S;
-      const bool ok = q < end;
-      const O qc = static_cast<O>(ok ? q : last);
-      const IT ci = elem<OFF32>(indices, qc);
+      const O qc = static_cast<O>(q < end ? q : last);
+      c[u] = elem<OFF32>(indices, qc);
       const O qv = PERM ? static_cast<O>(elem<OFF32>(perm, qc)) : qc;
-      const A ai = static_cast<A>(elem<OFF32>(data, qv));
-      c[u] = ok ? ci : IT(0);
-      a[u] = ok ? ai : A(0);
+      a[u] = elem<OFF32>(data, qv);
     }
   };
 
-  int64_t rr = i_begin + wave * per_wave + sub;
+  int64_t rr = tr.first * per_block + wave * per_wave + sub;
   O out;
   P beg, end;
   item(rr, out, beg, end);
   IT c[UNROLL];
-  A a[UNROLL];
+  T a[UNROLL];
   trip(beg + slot, end, c, a);
 
-  for (int64_t base = i_begin; base < i_end; base += per_block) {  // wave-uniform
+  for (int64_t tile = tr.first; tile < tr.end; tile += tr.step) {  // wave-uniform
     const bool active = rr < lb.count && col_ok;
-    const int64_t rr_next = rr + per_block;
+    const int64_t rr_next = rr + rr_step;
     O out_n;
     P beg_n, end_n;
     item(rr_next, out_n, beg_n, end_n);
@@ -124,16 +125,17 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
       const P pn = p + static_cast<P>(S) * UNROLL;
       const bool same = pn < end;
       prefetched = prefetched || !same;
-      A an[UNROLL];
+      T an[UNROLL];
       IT cn[UNROLL];
       trip(same ? pn : beg_n + slot, same ? end : end_n, cn, an);
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
         const bool ok = p + static_cast<P>(u) * S < end;
+        const A av = ok ? static_cast<A>(a[u]) : A(0);
 #pragma unroll
         for (int v = 0; v < VEC; ++v) {
           const A bv = ok ? static_cast<A>(b[u].v[v]) : A(0);
-          acc[v] += a[u] * bv;
+          acc[v] += av * bv;
         }
       }
 #pragma unroll

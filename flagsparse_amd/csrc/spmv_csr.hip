@@ -28,8 +28,8 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmv_bins_kernel(
   const int l = lane & (L - 1);
   const int64_t per_wave = 64 >> ll;
   const int64_t per_block = per_wave * FSP_WAVES;
-  const int64_t t_begin = (static_cast<int64_t>(blockIdx.x) - lb.block_begin) * lb.chunk;
-  for (int64_t tile = t_begin; tile < t_begin + lb.chunk && tile * per_block < lb.count; ++tile) {
+  const tile_range tr = bin_tiles(lb, per_block);
+  for (int64_t tile = tr.first; tile < tr.end; tile += tr.step) {
     const int64_t rr = tile * per_block + wave * per_wave + sub;
     const bool active = rr < lb.count;
     A acc = A(0);

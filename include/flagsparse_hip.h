@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSP_ABI_VERSION 3
+#define FSP_ABI_VERSION 4
 #define FSP_MAX_BINS 8
 
 typedef enum fsp_value_type {
@@ -77,6 +77,10 @@ typedef struct fsp_csr_plan {
   int64_t max_blocks;             /* 0: one workgroup per row tile; > 0: persistent
                                      grid of at most this many workgroups, each
                                      walking a contiguous run of tiles */
+  int64_t xcd_groups;             /* > 1: cut each bin's row tiles into this many
+                                     contiguous regions, workgroup b serving
+                                     region b % xcd_groups (8 = one per XCD
+                                     on MI355X: neighbouring rows share an L2) */
 } fsp_csr_plan;
 
 /* ---- library ------------------------------------------------------------ */

@@ -31,9 +31,10 @@ def test_shape_class_buckets():
 
 def test_candidates_and_configs():
     sv = autotune.candidates("spmv")
-    assert len(sv) == len(set(sv)) == 9
+    assert len(sv) == len(set(sv)) == 18
     sm = autotune.candidates("spmm", 32, 4)
-    assert len(sm) == len(set(sm)) == 24
+    assert len(sm) == len(set(sm)) == 48
+    assert {p.xcd_groups for p in sm} == {0, 8}
     assert {p.lanes_n for p in sm} == {8, 4}
     assert {p.lanes_n for p in autotune.candidates("spmm", 8, 4)} == {2}
     p = autotune.TuneParams(unroll=8, slot_scale=2, long_threshold=512)

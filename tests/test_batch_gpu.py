@@ -107,17 +107,17 @@ def test_batch_long_rows_split_into_segments(device):
         assert rel_err(C[r0:r1], O.spmm(ip, ix, v, B[c0:c1], shape)) <= 1e-5
 
 
-@pytest.mark.parametrize("bpc", [1, 2, 8])
-def test_persistent_grid_is_bitwise_equal(device, bpc):
+@pytest.mark.parametrize("bpc,xg", [(1, 0), (2, 0), (8, 0), (0, 8), (1, 8), (8, 8), (8, 3)])
+def test_persistent_and_xcd_grids_are_bitwise_equal(device, bpc, xg):
     ip, ix, v, ro, no = S.csr_batch(n_mats=96, rows_lo=16, rows_hi=2048, seed=11, device=device)
     b = CsrBatch(v, ix, ip, ro, None, no)
     x = torch.rand(b.total_cols, device=device) - 0.5
     B = torch.randn(b.total_cols, 32, device=device)
     base_v = autotune.TuneParams(unroll=4, blocks_per_cu=0)
-    pers_v = autotune.TuneParams(unroll=4, blocks_per_cu=bpc)
+    pers_v = autotune.TuneParams(unroll=4, blocks_per_cu=bpc, xcd_groups=xg)
     assert torch.equal(b.spmv(x, params=base_v), b.spmv(x, params=pers_v))
     base_m = autotune.TuneParams(unroll=4, slot_scale=2, blocks_per_cu=0)
-    pers_m = autotune.TuneParams(unroll=4, slot_scale=2, blocks_per_cu=bpc)
+    pers_m = autotune.TuneParams(unroll=4, slot_scale=2, blocks_per_cu=bpc, xcd_groups=xg)
     assert torch.equal(b.spmm(B, params=base_m), b.spmm(B, params=pers_m))
 
 
