@@ -104,7 +104,7 @@ class CsrHandle:
         )
         return C
 
-    def spmv(self, data, x, transpose=False, out=None, config=None, max_blocks=0, xcd_groups=0):
+    def spmv(self, data, x, transpose=False, out=None, config=None, max_blocks=0, xcd_groups=0, variant=0):
         n_out = self.n_cols if transpose else self.n_rows
         y = out if out is not None else torch.empty(n_out, dtype=data.dtype, device=x.device)
         if n_out == 0:
@@ -119,7 +119,7 @@ class CsrHandle:
             ptr_t, idx_t, perm_t = self.indptr, self.indices, None
         acc = torch.float64 if data.dtype in (torch.float32, torch.float64) else torch.float32
         partials = plan.partials(1, acc, x.device)
-        st = plan.struct(partials, max_blocks, xcd_groups)
+        st = plan.struct(partials, max_blocks, xcd_groups, variant)
         _lib.check(
             _lib.load().fsp_spmv_csr(
                 _lib.value_code(data.dtype),
@@ -133,6 +133,7 @@ class CsrHandle:
                 _lib.ptr(x),
                 y.data_ptr(),
                 n_out,
+                self.nnz,
                 st,
                 _lib.stream_ptr(x.device),
             ),

@@ -157,17 +157,18 @@ class CsrPlan:
             self._partials[key] = buf
         return buf
 
-    def struct(self, partials=None, max_blocks=0, xcd_groups=0):
+    def struct(self, partials=None, max_blocks=0, xcd_groups=0, variant=0):
         """The C plan.  max_blocks > 0 makes the launch persistent: at most
         that many workgroups, each walking a contiguous run of row tiles;
         xcd_groups > 1 cuts each bin's tiles into that many contiguous regions
         served by the workgroups b with b % xcd_groups == region (8 = one
-        region per XCD)."""
+        region per XCD).  variant selects the SpMV loop (0 pipelined, 1 plain)."""
         st = _lib.FspCsrPlan()
         st.n_bins = len(self.bin_counts)
         st.unroll = self.config.unroll
         st.max_blocks = int(max_blocks)
         st.xcd_groups = int(xcd_groups)
+        st.variant = int(variant)
         off = 0
         for i, (lanes, count) in enumerate(zip(self.config.lanes, self.bin_counts)):
             st.bins[i].lanes = int(lanes)

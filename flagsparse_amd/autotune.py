@@ -14,7 +14,8 @@ The tuned parameters are the ones the kernels take from their plan:
                      blocks_per_cu workgroups per compute unit, each walking a
                      contiguous run of rows;
 * ``xcd_groups``     8 = cut each row bin into 8 contiguous regions, one per
-                     XCD, so neighbouring rows share an L2 (0 = off).
+                     XCD, so neighbouring rows share an L2 (0 = off);
+* ``variant``        SpMV loop: 0 = software-pipelined, 1 = plain.
 
 A shape class is (kernel, value / index dtypes, dense width, rounded log2 of
 the mean and maximum row length and of the row count, compute units).  The
@@ -50,6 +51,7 @@ class TuneParams:
     long_threshold: int = 0  # 0 = the kernel family's default
     blocks_per_cu: int = 0
     xcd_groups: int = 0
+    variant: int = 0
 
     def spmv_config(self):
         lt = self.long_threshold or _plan.SPMV_LONG_THRESHOLD
@@ -94,10 +96,11 @@ def candidates(kind, n_dense=1, elem_bytes=4):
     """Candidate parameter sets of a kernel family (small, fixed grids)."""
     out = []
     if kind == "spmv":
-        for unroll in (2, 4, 8):
-            for bpc in (0, 4, 8):
-                for xg in (0, 8):
-                    out.append(TuneParams(unroll=unroll, blocks_per_cu=bpc, xcd_groups=xg))
+        for variant in (0, 1):
+            for unroll in (2, 4, 8):
+                for bpc in (0, 4, 8):
+                    for xg in (0, 8):
+                        out.append(TuneParams(unroll=unroll, blocks_per_cu=bpc, xcd_groups=xg, variant=variant))
         return out
     g = _plan.lanes_for_dense(n_dense, elem_bytes)
     lanes = [g] + ([g // 2] if g >= 8 else [])
@@ -187,6 +190,32 @@ def clear_memory():
 
 
 # ------------------------------------------------------------------ tuning
+def tune_spmv(handle, data, allow_tuning=True):
+    """Parameters of y = A x for a CsrHandle (shape class of its rows): the
+    cached winner, or tuned now on this matrix (x = 1, y scratch)."""
+    device = data.device
+    lengths = handle.indptr[1:] - handle.indptr[:-1] if handle.n_rows else handle.indptr[:0]
+    max_row = int(lengths.max().item()) if handle.n_rows else 0
+    key = shape_class("spmv", data.dtype, handle.indices.dtype, handle.indptr.dtype, 1, handle.n_rows, handle.nnz,
+                      max_row, compute_units(device))
+    hit = lookup(key)
+    if hit is not None or not allow_tuning:
+        return hit or TuneParams()
+    x = torch.ones(handle.n_cols, dtype=data.dtype, device=device)
+    y = torch.empty(handle.n_rows, dtype=data.dtype, device=device)
+    cands = candidates("spmv")
+    for p in cands:  # plans (host syncs) before any timing
+        handle.plan(p.spmv_config())
+
+    def run(p):
+        handle.spmv(data, x, out=y, config=p.spmv_config(), max_blocks=p.max_blocks(device),
+                    xcd_groups=p.xcd_groups, variant=p.variant)
+
+    best, _ = tune(key, cands, run)
+    return best
+
+
+
 def time_launch(fn, reps=5):
     """Median device time (ms) of fn() on the current stream after a warm-up."""
     fn()

@@ -161,7 +161,7 @@ class CsrBatch:
 
             def run(p):
                 self.handle.spmv(data, x, out=y, config=p.spmv_config(), max_blocks=p.max_blocks(self.device),
-                                 xcd_groups=p.xcd_groups)
+                                 xcd_groups=p.xcd_groups, variant=p.variant)
         else:
             B = torch.ones((self.total_cols, n_dense), dtype=dtype, device=self.device)
             C = torch.empty((self.total_rows, n_dense), dtype=dtype, device=self.device)
@@ -194,7 +194,7 @@ class CsrBatch:
             raise ValueError("x must be 1D with the values' dtype")
         p = params or self.params("spmv", 1, data.dtype)
         return self.handle.spmv(data, x.contiguous(), out=out, config=p.spmv_config(),
-                                max_blocks=p.max_blocks(self.device), xcd_groups=p.xcd_groups)
+                                max_blocks=p.max_blocks(self.device), xcd_groups=p.xcd_groups, variant=p.variant)
 
     def spmm(self, B, out=None, data=None, params=None):
         """C (packed by rows) = blockdiag(A_m) B (packed by rows): one launch."""
@@ -244,7 +244,7 @@ class CsrBatch:
         y = out if out is not None else torch.empty(self.total_rows, dtype=data.dtype, device=self.device)
         cfg = p.spmv_config()
         for h, r0, r1, n0, n1, c0, c1 in self._matrices():
-            h.spmv(data[n0:n1], x[c0:c1], out=y[r0:r1], config=cfg)
+            h.spmv(data[n0:n1], x[c0:c1], out=y[r0:r1], config=cfg, variant=p.variant)
         return y
 
     def spmm_per_matrix(self, B, out=None, data=None, params=None):

@@ -130,11 +130,15 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void sddmm_bins_kernel(
     trip(beg + slot, end, c);
     for (int64_t tile = tr.first; tile < tr.end; tile += tr.step) {  // wave-uniform
       const int64_t rr_next = rr + rr_step;
-      O row_n;
-      P beg_n, end_n;
-      item(rr_next, row_n, beg_n, end_n);
-      const V xv_n =
-          *addr<OFF32>::at(reinterpret_cast<const V*>(X), (row_n * ldx_o + kk_ld) * static_cast<O>(sizeof(T)));
+      // the next item is prefetched only when this workgroup will process it
+      const bool has_next = tile + tr.step < tr.end;  // wave-uniform
+      O row_n = 0;
+      P beg_n = 0, end_n = 0;
+      V xv_n{};
+      if (has_next) {
+        item(rr_next, row_n, beg_n, end_n);
+        xv_n = *addr<OFF32>::at(reinterpret_cast<const V*>(X), (row_n * ldx_o + kk_ld) * static_cast<O>(sizeof(T)));
+      }
       A xr[VEC];
 #pragma unroll
       for (int v = 0; v < VEC; ++v) xr[v] = k_ok ? static_cast<A>(xv.v[v]) : A(0);
@@ -150,7 +154,7 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void sddmm_bins_kernel(
         const bool same = pn < end;
         prefetched = prefetched || !same;
         IT cn[UNROLL];
-        trip(same ? pn : beg_n + slot, same ? end : end_n, cn);
+        if (same || has_next) trip(same ? pn : beg_n + slot, same ? end : end_n, cn);
         A dot[UNROLL];
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
@@ -163,7 +167,7 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void sddmm_bins_kernel(
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) c[u] = cn[u];
       }
-      if (!prefetched) trip(beg_n + slot, end_n, c);
+      if (!prefetched && has_next) trip(beg_n + slot, end_n, c);
       xv = xv_n;
       row = row_n;
       beg = beg_n;

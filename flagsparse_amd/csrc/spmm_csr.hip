@@ -107,9 +107,11 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
   for (int64_t tile = tr.first; tile < tr.end; tile += tr.step) {  // wave-uniform
     const bool active = rr < lb.count && col_ok;
     const int64_t rr_next = rr + rr_step;
-    O out_n;
-    P beg_n, end_n;
-    item(rr_next, out_n, beg_n, end_n);
+    // the next item is prefetched only when this workgroup will process it
+    const bool has_next = tile + tr.step < tr.end;  // wave-uniform
+    O out_n = 0;
+    P beg_n = 0, end_n = 0;
+    if (has_next) item(rr_next, out_n, beg_n, end_n);
     A acc[VEC];
 #pragma unroll
     for (int v = 0; v < VEC; ++v) acc[v] = A(0);
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
       prefetched = prefetched || !same;
       T an[UNROLL];
       IT cn[UNROLL];
-      trip(same ? pn : beg_n + slot, same ? end : end_n, cn, an);
+      if (same || has_next) trip(same ? pn : beg_n + slot, same ? end : end_n, cn, an);
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
         const bool ok = p + static_cast<P>(u) * S < end;
@@ -144,7 +146,7 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
         a[u] = an[u];
       }
     }
-    if (!prefetched) trip(beg_n + slot, end_n, c, a);
+    if (!prefetched && has_next) trip(beg_n + slot, end_n, c, a);
     // Sum the S slots of a row: lanes that differ only in the slot bits.
     for (int j = 0; j < s_log2; ++j) {
 #pragma unroll

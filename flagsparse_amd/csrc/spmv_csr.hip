@@ -3,7 +3,10 @@
 // Rows are binned by length; a bin with lanes L puts 64/L rows on one wave and
 // splits each row's nonzeros over L lanes (lane l takes nonzeros l, l+L, ...),
 // UNROLL independent (index, value, x) loads in flight per lane, then a DPP
-// group sum.  Rows longer than the last bin are cut into segments; one wave
+// group sum.  Software-pipelined like the SpMM: while a trip's x gathers are
+// in flight a lane loads the column ids / values of its next trip -- of the
+// same row, or the first trip of its next row (whose row pointer was loaded
+// one row earlier).  Rows longer than the last bin are cut into segments; one wave
 // reduces each segment into a partial and a second pass adds a row's partials
 // in segment order (no atomics: the result is deterministic for a plan).
 #include "fsp_device.hpp"
@@ -14,6 +17,105 @@ namespace fsp {
 
 template <typename T, typename IT, typename PT, typename QT, int UNROLL, bool PERM>
 __global__ __launch_bounds__(FSP_WG_THREADS) void spmv_bins_kernel(
+    const PT* __restrict__ indptr, const IT* __restrict__ indices, const T* __restrict__ data,
+    const QT* __restrict__ perm, const T* __restrict__ x, T* __restrict__ y,
+    typename spmv_acc_of<T>::type* __restrict__ partials, launch_table tab) {
+  using A = typename spmv_acc_of<T>::type;
+  const int k = find_bin(tab, blockIdx.x);
+  const launch_bin& lb = tab.b[k];
+  const int ll = lb.lanes_log2;
+  const int L = 1 << ll;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int sub = lane >> ll;
+  const int l = lane & (L - 1);
+  const int64_t per_wave = 64 >> ll;
+  const int64_t per_block = per_wave * FSP_WAVES;
+  const tile_range tr = bin_tiles(lb, per_block);
+  const int64_t rr_step = tr.step * per_block;
+  const int64_t last_item = lb.count - 1;  // bins in the table are non-empty
+
+  // Item of this lane: output slot and nonzero range (empty past the bin).
+  auto item = [&](int64_t r, int64_t& out, int64_t& beg, int64_t& end) {
+    const bool live = r < lb.count;
+    const work_item w = load_item(tab, lb, indptr, live ? r : last_item);
+    out = w.out;
+    beg = live ? w.begin : 0;
+    end = live ? w.end : 0;
+  };
+  // Column ids and values of nonzeros p, p+L, ...; past `end` the loads repeat
+  // the item's last nonzero.  Raw: the mask is applied at the FMA, so the
+  // prefetch does not make the compiler wait for the x gathers in flight.
+  auto trip = [&](int64_t p, int64_t end, IT (&c)[UNROLL], T (&a)[UNROLL]) {
+    const int64_t lastnz = end > 0 ? end - 1 : 0;
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      const int64_t q = p + static_cast<int64_t>(u) * L;
+      const int64_t qc = q < end ? q : lastnz;
+      c[u] = indices[qc];
+      a[u] = data[PERM ? static_cast<int64_t>(perm[qc]) : qc];
+    }
+  };
+
+  int64_t rr = tr.first * per_block + wave * per_wave + sub;
+  int64_t out, beg, end;
+  item(rr, out, beg, end);
+  IT c[UNROLL];
+  T a[UNROLL];
+  trip(beg + l, end, c, a);
+  for (int64_t tile = tr.first; tile < tr.end; tile += tr.step) {  // wave-uniform
+    const bool active = rr < lb.count;
+    const int64_t rr_next = rr + rr_step;
+    // the next item is prefetched only when this workgroup will process it
+    const bool has_next = tile + tr.step < tr.end;  // wave-uniform
+    int64_t out_n = 0, beg_n = 0, end_n = 0;
+    if (has_next) item(rr_next, out_n, beg_n, end_n);
+    A acc = A(0);
+    bool prefetched = false;
+    for (int64_t p = beg + l; p < end; p += static_cast<int64_t>(L) * UNROLL) {
+      T xv[UNROLL];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) xv[u] = x[static_cast<int64_t>(c[u])];
+      // prefetch: the next trip of this item, else the first trip of the next
+      const int64_t pn = p + static_cast<int64_t>(L) * UNROLL;
+      const bool same = pn < end;
+      prefetched = prefetched || !same;
+      IT cn[UNROLL];
+      T an[UNROLL];
+      if (same || has_next) trip(same ? pn : beg_n + l, same ? end : end_n, cn, an);
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        const bool ok = p + static_cast<int64_t>(u) * L < end;
+        acc += ok ? static_cast<A>(a[u]) * static_cast<A>(xv[u]) : A(0);
+      }
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        c[u] = cn[u];
+        a[u] = an[u];
+      }
+    }
+    if (!prefetched && has_next) trip(beg_n + l, end_n, c, a);
+    acc = group_sum(acc, L);
+    if (active && l == 0) {
+      if (lb.is_segment) {
+        partials[out] = acc;
+      } else {
+        y[out] = static_cast<T>(acc);
+      }
+    }
+    rr = rr_next;
+    out = out_n;
+    beg = beg_n;
+    end = end_n;
+  }
+}
+
+// The plain loop (plan.variant = 1): every trip loads its column ids / values
+// and then gathers x, with no prefetch across trips.  For short uniform rows
+// under a persistent grid it streams the index / value arrays faster than the
+// pipelined loop (measured on MI355X); the autotuner picks between the two.
+template <typename T, typename IT, typename PT, typename QT, int UNROLL, bool PERM>
+__global__ __launch_bounds__(FSP_WG_THREADS) void spmv_simple_kernel(
     const PT* __restrict__ indptr, const IT* __restrict__ indices, const T* __restrict__ data,
     const QT* __restrict__ perm, const T* __restrict__ x, T* __restrict__ y,
     typename spmv_acc_of<T>::type* __restrict__ partials, launch_table tab) {
@@ -68,6 +170,14 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmv_bins_kernel(
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(FSP_WG_THREADS) void spmv_zero_kernel(T* __restrict__ y, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * FSP_WG_THREADS + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * FSP_WG_THREADS) {
+    y[i] = T(0);
+  }
+}
+
 // y[long_rows[i]] = sum of the row's segment partials, in segment order.
 template <typename T>
 __global__ __launch_bounds__(FSP_WG_THREADS) void spmv_long_reduce_kernel(
@@ -99,10 +209,17 @@ static int launch_spmv(const void* indptr, const void* indices, const void* data
   if (blocks > 0) {
     const int unroll = plan->unroll;
 #define FSP_SPMV_LAUNCH(U)                                                                          \
-  hipLaunchKernelGGL((spmv_bins_kernel<T, IT, PT, QT, U, PERM>), dim3(blocks), dim3(FSP_WG_THREADS), 0, s, \
-                     static_cast<const PT*>(indptr), static_cast<const IT*>(indices),                \
-                     static_cast<const T*>(data), static_cast<const QT*>(perm),                      \
-                     static_cast<const T*>(x), static_cast<T*>(y), partials, tab)
+  if (plan->variant == 1) {                                                                          \
+    hipLaunchKernelGGL((spmv_simple_kernel<T, IT, PT, QT, U, PERM>), dim3(blocks), dim3(FSP_WG_THREADS), 0, \
+                       s, static_cast<const PT*>(indptr), static_cast<const IT*>(indices),          \
+                       static_cast<const T*>(data), static_cast<const QT*>(perm),                    \
+                       static_cast<const T*>(x), static_cast<T*>(y), partials, tab);                 \
+  } else {                                                                                           \
+    hipLaunchKernelGGL((spmv_bins_kernel<T, IT, PT, QT, U, PERM>), dim3(blocks), dim3(FSP_WG_THREADS), 0, s, \
+                       static_cast<const PT*>(indptr), static_cast<const IT*>(indices),              \
+                       static_cast<const T*>(data), static_cast<const QT*>(perm),                    \
+                       static_cast<const T*>(x), static_cast<T*>(y), partials, tab);                 \
+  }
     if (unroll <= 2) {
       FSP_SPMV_LAUNCH(2);
     } else if (unroll <= 4) {
@@ -131,13 +248,25 @@ using namespace fsp;
 extern "C" int fsp_spmv_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
                             const void* indptr_dev, const void* indices_dev, const void* data_dev,
                             int32_t perm_type, const void* perm_dev, const void* x_dev, void* y_dev,
-                            int64_t n_rows, const fsp_csr_plan* plan, void* stream) {
-  if (n_rows < 0) return fail(FSP_ERR_ARG, "fsp_spmv_csr: negative n_rows");
+                            int64_t n_rows, int64_t nnz, const fsp_csr_plan* plan, void* stream) {
+  if (n_rows < 0 || nnz < 0) return fail(FSP_ERR_ARG, "fsp_spmv_csr: negative n_rows or nnz");
   int st = check_plan(plan, n_rows);
   if (st) return st;
   if (n_rows == 0) return FSP_OK;
   if (indptr_dev == nullptr || y_dev == nullptr) return fail(FSP_ERR_ARG, "fsp_spmv_csr: NULL indptr or y");
+  if (nnz > 0 && (indices_dev == nullptr || data_dev == nullptr || x_dev == nullptr)) {
+    return fail(FSP_ERR_ARG, "fsp_spmv_csr: NULL indices, data or x");
+  }
   const hipStream_t s = as_stream(stream);
+  if (nnz == 0) {  // y = 0: the row kernels read the first nonzero unconditionally
+    FSP_DISPATCH_VALUE(value_type, T, {
+      int64_t g = ceil_div(n_rows, FSP_WG_THREADS);
+      if (g > 65536) g = 65536;
+      hipLaunchKernelGGL(spmv_zero_kernel<T>, dim3(static_cast<unsigned>(g)), dim3(FSP_WG_THREADS), 0, s,
+                         static_cast<T*>(y_dev), n_rows);
+      return check_launch("spmv_zero");
+    });
+  }
   const bool use_perm = perm_dev != nullptr;
   FSP_DISPATCH_VALUE(value_type, T, {
     FSP_DISPATCH_INDEX(index_type, IT, {

@@ -295,6 +295,7 @@ def _hip_spmv_csr_impl_prepared(prepared, x, out=None):
             _lib.ptr(x),
             y.data_ptr(),
             prepared.n_rows,
+            int(prepared.kernel_indices.numel()),
             st,
             _lib.stream_ptr(device),
         ),

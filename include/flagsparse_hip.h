@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSP_ABI_VERSION 4
+#define FSP_ABI_VERSION 6
 #define FSP_MAX_BINS 8
 
 typedef enum fsp_value_type {
@@ -81,6 +81,9 @@ typedef struct fsp_csr_plan {
                                      contiguous regions, workgroup b serving
                                      region b % xcd_groups (8 = one per XCD
                                      on MI355X: neighbouring rows share an L2) */
+  int64_t variant;                /* SpMV loop: 0 = software-pipelined (next trip's
+                                     indices load under this trip's x gathers),
+                                     1 = plain loop; other kernels ignore it */
 } fsp_csr_plan;
 
 /* ---- library ------------------------------------------------------------ */
@@ -130,11 +133,12 @@ int fsp_gather(int32_t elem_bytes, int32_t perm_type, const void* perm_dev,
  * _triton_spmv_csr_impl_prepared (spmv_csr.py:702-746) and the bucketed
  * kernels of the use_opt path (spmv_csr.py:247-368, 423-482).
  * Accumulation: f64 for f32/f64 values (the reference promotes f32 SpMV to
- * f64, spmv_csr.py:138-143), f32 for f16/bf16.  y has value_type. */
+ * f64, spmv_csr.py:138-143), f32 for f16/bf16.  y has value_type.  nnz = 0
+ * writes y = 0 (indices / data / x may then be NULL). */
 int fsp_spmv_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
                  const void* indptr_dev, const void* indices_dev,
                  const void* data_dev, int32_t perm_type, const void* perm_dev,
-                 const void* x_dev, void* y_dev, int64_t n_rows,
+                 const void* x_dev, void* y_dev, int64_t n_rows, int64_t nnz,
                  const fsp_csr_plan* plan, void* stream);
 
 /* ---- SpMM ----------------------------------------------------------------

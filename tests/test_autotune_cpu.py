@@ -31,7 +31,8 @@ def test_shape_class_buckets():
 
 def test_candidates_and_configs():
     sv = autotune.candidates("spmv")
-    assert len(sv) == len(set(sv)) == 18
+    assert len(sv) == len(set(sv)) == 36
+    assert {p.variant for p in sv} == {0, 1}
     sm = autotune.candidates("spmm", 32, 4)
     assert len(sm) == len(set(sm)) == 48
     assert {p.xcd_groups for p in sm} == {0, 8}

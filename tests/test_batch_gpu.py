@@ -218,3 +218,18 @@ def test_full_batch(device, full_batch):
     assert torch.equal(b.spmv_per_matrix(x, params=pv), y)
     assert torch.equal(b.spmm_per_matrix(B, params=pm), C)
     _oracle_sample(b, arrays, x, B, y, C)
+
+
+def test_tune_spmv_single_matrix(device):
+    from flagsparse_amd._matrix import CsrHandle
+
+    ip, ix, v, _ = S.power_law_rows(30000, 30000, 2.1, 24, 4096, n_dense=0, seed=8)
+    data, idx, ptr = S.to_torch(ip, ix, v, device)
+    h = CsrHandle(idx, ptr, (30000, 30000))
+    p = autotune.tune_spmv(h, data)
+    assert p in autotune.candidates("spmv")
+    assert autotune.tune_spmv(h, data) == p  # cached for the shape class
+    x = np.random.default_rng(1).uniform(-1, 1, 30000)
+    y = h.spmv(data, S.dense(x, device), config=p.spmv_config(), max_blocks=p.max_blocks(device),
+               xcd_groups=p.xcd_groups, variant=p.variant)
+    assert rel_err(y, O.spmv(ip, ix, v, x, (30000, 30000))) <= 1e-5

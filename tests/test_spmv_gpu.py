@@ -231,3 +231,25 @@ def test_spmv_int64_auto_does_not_fallback_when_index_exceeds_int32(device, monk
     x = torch.empty(0, dtype=torch.float32, device=device)
     with pytest.raises(RuntimeError, match="int32 fallback is unsafe"):
         spmv_mod._run_spmv_prepared_with_fallback(prepared, x, use_opt=False)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("max_blocks,xcd", [(0, 0), (512, 0), (0, 8), (1024, 8)])
+def test_spmv_loop_variants_and_grids_bitwise_equal(device, variant, max_blocks, xcd):
+    """The pipelined (0) and plain (1) SpMV loops, one-tile-per-workgroup,
+    persistent and XCD-region grids all reduce every row in the same order."""
+    from flagsparse_amd import _plan
+    from flagsparse_amd._matrix import CsrHandle
+
+    ip, ix, v = long_row_matrix(5000, 3000, long_rows=(7, 4000), long_len=2900, short_len=9, seed=21)
+    pl_ip, pl_ix, pl_v, _ = synthetic.power_law_rows(20000, 20000, 2.1, 24, 4096, n_dense=0, seed=5)
+    for ip_, ix_, v_, shape in ((ip, ix, v, (5000, 3000)), (pl_ip, pl_ix, pl_v, (20000, 20000))):
+        data, idx, ptr = device_csr(ip_, ix_, v_, device)
+        h = CsrHandle(idx, ptr, shape)
+        x = np.random.default_rng(3).uniform(-1, 1, shape[1])
+        xt = synthetic.dense(x, device)
+        cfg = _plan.spmv_config(unroll=4)
+        ref = h.spmv(data, xt, config=cfg)
+        got = h.spmv(data, xt, config=cfg, max_blocks=max_blocks, xcd_groups=xcd, variant=variant)
+        assert torch.equal(got, ref)
+        assert_rel(got, oracle.spmv(ip_, ix_, v_, x, shape), torch.float32, "spmv variant")
