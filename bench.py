@@ -386,6 +386,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="headline SpMM only (profiling runs)")
     ap.add_argument("--with-copy", action="store_true",
                     help="also run the 1 GiB copy (the counter calibration kernel) under --no-extras")
+    ap.add_argument("--only", default="",
+                    help="comma-separated extras to run (copy, spmv_large, spmv_small, autograd, bsr, batch); "
+                         "default: all")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the hot path is a single-GPU kernel); got --gpus {args.gpus}",
@@ -441,19 +444,26 @@ def main():
             "ms_per_step": round(r["trans_ms"], 5),
         },
     }
-    if args.no_extras and args.with_copy:
+    extras = set(e for e in args.only.split(",") if e) if args.only else {
+        "copy", "spmv_large", "spmv_small", "autograd", "bsr", "batch"}
+    if args.no_extras:
+        extras = {"copy"} if args.with_copy else set()
+    if "copy" in extras:
         line["hbm_copy_1gib"] = bench_copy(dev, args.steps)
-    if not args.no_extras:
-        line["hbm_copy_1gib"] = bench_copy(dev, args.steps)
+    if "spmv_large" in extras:
         line["spmv_uniform_16m_rows_x16"] = bench_spmv_large(dev, args.steps, args.warmup)
+    if "spmv_small" in extras:
         s = bench_spmv_uniform(dev, args.steps, args.warmup)
         line["spmv_uniform_16384x16"] = {"ms_per_step": round(s["ms"], 5),
                                          "event_ms_median": round(s["event_ms_median"], 5),
                                          "gbs_cache_resident": round(s["gbs_cache_resident"], 2)}
+    if "autograd" in extras:
         a = bench_autograd_bf16_vs_fp32(dev, max(5, args.steps // 5), args.warmup)
         line["bf16_fwd_bwd_speedup_vs_fp32"] = round(a["speedup_bf16_vs_fp32"], 3)
         line["autograd_200k"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in a.items()}
+    if "bsr" in extras:
         line["bsr_65536_b40_n128"] = bench_bsr(dev, args.steps, args.warmup)
+    if "batch" in extras:
         line["batch_4096"] = bench_batch(dev, args.steps, args.warmup)
     if not args.no_cpu_baseline:
         log("cpu baseline")

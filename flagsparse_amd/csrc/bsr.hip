@@ -293,16 +293,19 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void bsr_quantize_fp8_kernel(const 
 // block j, 32-63 block j+1), so each C element is summed by one lane in block
 // order (deterministic, no cross-wave reduction, no LDS).  The next pair's A
 // and B fragments are loaded while the current pair's MFMAs run.
-constexpr int kTilesPerWave = kNt / FSP_WAVES;  // 2
-
-template <int KIND, typename BP>
-__global__ __launch_bounds__(FSP_WG_THREADS) void bsr_spmm_kernel(
+// WPG = waves per workgroup; each wave owns kNt / WPG tiles.  Every wave
+// re-reads the block row's A fragments, so fewer, wider waves pay off when B
+// is cheap to move (fp8: 2 waves x 64 columns; bf16 / f16: 4 x 32; measured
+// on MI355X).
+template <int KIND, typename BP, int WPG>
+__global__ __launch_bounds__(64 * WPG) void bsr_spmm_kernel(
     const BP* __restrict__ block_ptr, const int32_t* __restrict__ block_col, const void* __restrict__ block_data,
     const float* __restrict__ block_scales, const void* __restrict__ Bp, const float* __restrict__ b_absmax,
     int64_t nbc, int64_t Npad, int64_t N, float* __restrict__ C, int64_t ldc, int64_t M) {
   using V = typename std::conditional<KIND == 0, bf16x8, f16x8>::type;
   using E = typename std::conditional<KIND == 0, __bf16, _Float16>::type;
   using AF = typename std::conditional<KIND == 2, uint64_t, V>::type;  // one lane's A (or B) fragment
+  constexpr int kTilesPerWave = kNt / WPG;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int64_t br = blockIdx.x;
@@ -586,13 +589,13 @@ int fsp_bsr_spmm(int32_t value_type, int32_t block_ptr_type, const void* block_p
   FSP_DISPATCH_INDEX(block_ptr_type, BP, {
     const BP* bp = static_cast<const BP*>(block_ptr_dev);
     if (value_type == FSP_BF16) {
-      hipLaunchKernelGGL((bsr_spmm_kernel<0, BP>), grid, dim3(FSP_WG_THREADS), 0, s, bp, block_col_dev, block_data_dev,
+      hipLaunchKernelGGL((bsr_spmm_kernel<0, BP, 4>), grid, dim3(64 * 4), 0, s, bp, block_col_dev, block_data_dev,
                          nullptr, Bp_dev, nullptr, nbc, Npad, N, C_dev, ldc, M);
     } else if (value_type == FSP_F16) {
-      hipLaunchKernelGGL((bsr_spmm_kernel<1, BP>), grid, dim3(FSP_WG_THREADS), 0, s, bp, block_col_dev, block_data_dev,
+      hipLaunchKernelGGL((bsr_spmm_kernel<1, BP, 4>), grid, dim3(64 * 4), 0, s, bp, block_col_dev, block_data_dev,
                          nullptr, Bp_dev, nullptr, nbc, Npad, N, C_dev, ldc, M);
     } else if (value_type == FSP_F8E4M3) {
-      hipLaunchKernelGGL((bsr_spmm_kernel<2, BP>), grid, dim3(FSP_WG_THREADS), 0, s, bp, block_col_dev, block_data_dev,
+      hipLaunchKernelGGL((bsr_spmm_kernel<2, BP, 2>), grid, dim3(64 * 2), 0, s, bp, block_col_dev, block_data_dev,
                          block_scales_dev, Bp_dev, b_absmax_dev, nbc, Npad, N, C_dev, ldc, M);
     } else {
       return fail(FSP_ERR_DTYPE, "fsp_bsr_spmm: value type %d has no MFMA path", (int)value_type);
