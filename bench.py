@@ -226,6 +226,44 @@ def bench_spmv_large(dev, steps, warmup):
             "gbs": round(nbytes / (med * 1e-3) / 1e9, 2), "hbm_frac": round(nbytes / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+def bench_vendor(dev, steps, warmup, inputs):
+    """rocSPARSE (torch.sparse.mm on a ROCm build of torch) on the same two
+    workloads: the 262144^2 power-law SpMM (n = 64, f32/i32) and the 2^24-row
+    x16 SpMV.  Device-event medians on the current stream, output allocation
+    included as torch.sparse.mm does it per call."""
+    ip, ix, v, B = inputs
+    n = 262144
+    data, idx, ptr = synthetic.to_torch(ip, ix, v, dev, torch.float32, torch.int32)
+    Bt = synthetic.dense(B, dev, torch.float32)
+    A = torch.sparse_csr_tensor(ptr, idx, data, size=(n, n))
+    stream = torch.cuda.current_stream(dev)
+    out = {}
+    try:
+        _, ev = time_steps(lambda: torch.sparse.mm(A, Bt), steps, warmup, stream)
+        med = statistics.median(ev)
+        nbytes = spmm_bytes(int(ix.size), n, n, 64)
+        out["spmm_power_law_262144_n64"] = {"event_ms_median": round(med, 5),
+                                            "gbs": round(nbytes / (med * 1e-3) / 1e9, 2)}
+    except RuntimeError as exc:
+        out["spmm_power_law_262144_n64"] = {"error": str(exc)[:200]}
+    del A, data, idx, ptr, Bt
+    m, k = 1 << 24, 16384
+    ptr, idx, data, x = synthetic.uniform_rows_device(m, k, 16, seed=0, device=dev)
+    A = torch.sparse_csr_tensor(ptr, idx, data, size=(m, k))
+    xv = x.unsqueeze(1)
+    try:
+        _, ev = time_steps(lambda: torch.sparse.mm(A, xv), steps, warmup, stream)
+        med = statistics.median(ev)
+        nbytes = m * 16 * 8 + (m + 1) * 4 + k * 4 + m * 4
+        out["spmv_uniform_16m_rows_x16"] = {"event_ms_median": round(med, 5),
+                                            "gbs": round(nbytes / (med * 1e-3) / 1e9, 2)}
+    except RuntimeError as exc:
+        out["spmv_uniform_16m_rows_x16"] = {"error": str(exc)[:200]}
+    del A, ptr, idx, data, x, xv
+    out["library"] = "rocSPARSE via torch.sparse.mm (torch " + torch.__version__ + ")"
+    return out
+
+
 def bench_bsr(dev, steps, warmup):
     """BSR 65536 x 65536 (16 x 16 blocks, 40 per block row, seed 2) times
     B 65536 x 128 on the matrix cores: bf16, f16 and fp8 e4m3 (per-block
@@ -387,7 +425,7 @@ def main():
     ap.add_argument("--with-copy", action="store_true",
                     help="also run the 1 GiB copy (the counter calibration kernel) under --no-extras")
     ap.add_argument("--only", default="",
-                    help="comma-separated extras to run (copy, spmv_large, spmv_small, autograd, bsr, batch); "
+                    help="comma-separated extras to run (copy, spmv_large, spmv_small, autograd, bsr, batch, vendor); "
                          "default: all")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -445,7 +483,7 @@ def main():
         },
     }
     extras = set(e for e in args.only.split(",") if e) if args.only else {
-        "copy", "spmv_large", "spmv_small", "autograd", "bsr", "batch"}
+        "copy", "spmv_large", "spmv_small", "autograd", "bsr", "batch", "vendor"}
     if args.no_extras:
         extras = {"copy"} if args.with_copy else set()
     if "copy" in extras:
@@ -465,6 +503,16 @@ def main():
         line["bsr_65536_b40_n128"] = bench_bsr(dev, args.steps, args.warmup)
     if "batch" in extras:
         line["batch_4096"] = bench_batch(dev, args.steps, args.warmup)
+    if "vendor" in extras:
+        vend = bench_vendor(dev, args.steps, args.warmup, r["inputs"])
+        vs = vend.get("spmm_power_law_262144_n64", {}).get("event_ms_median")
+        if vs:
+            vend["spmm_speedup_vs_rocsparse"] = round(vs / kern_ms, 3)
+        sl = line.get("spmv_uniform_16m_rows_x16", {}).get("event_ms_median")
+        vv = vend.get("spmv_uniform_16m_rows_x16", {}).get("event_ms_median")
+        if sl and vv:
+            vend["spmv_speedup_vs_rocsparse"] = round(vv / sl, 3)
+        line["rocsparse"] = vend
     if not args.no_cpu_baseline:
         log("cpu baseline")
         cb = cpu_baseline(r["inputs"])

@@ -22,12 +22,14 @@ from ._common import (
     error_metrics,
     validation_metrics,
 )
+from .benchmarks import benchmark_sddmm_case, benchmark_spmm_case, benchmark_spmv_case
 from .autograd import CsrMatrix, flagsparse_spmm_csr_autograd, spmm_csr_autograd
 from .bsr import csr_to_bsr, flagsparse_spmm_bsr, quantize_bsr_fp8
 from .sddmm_csr import SDDMMPrepared, flagsparse_sddmm_csr, prepare_sddmm_csr
 from .spmm_csr import (
     SPMM_CSR_ALGORITHMS,
     PreparedCsrSpmmOpt,
+    PreparedCsrSpmmOptAlg2,
     PreparedCsrSpmmRoute,
     SpmmCsrAlgorithm,
     SpmmCsrAlgorithmUnavailable,
@@ -35,11 +37,15 @@ from .spmm_csr import (
     flagsparse_spmm_csr_opt,
     flagsparse_spmm_csr_opt_alg1,
     flagsparse_spmm_csr_opt_alg1_preprocess,
+    flagsparse_spmm_csr_opt_alg2,
+    flagsparse_spmm_csr_opt_alg2_preprocess,
     flagsparse_spmm_csr_run,
     list_spmm_csr_algorithms,
     prepare_spmm_csr_opt,
     prepare_spmm_csr_opt_alg1,
     prepare_spmm_csr_opt_alg1_preprocess,
+    prepare_spmm_csr_opt_alg2,
+    prepare_spmm_csr_opt_alg2_preprocess,
     prepare_spmm_csr_route,
     resolve_spmm_csr_algorithm,
 )
@@ -51,6 +57,9 @@ from .spmv_csr import PreparedCsrSpmv, flagsparse_spmv_csr, prepare_spmv_csr
 __version__ = "0.1.0"
 
 __all__ = [
+    "benchmark_spmv_case",
+    "benchmark_spmm_case",
+    "benchmark_sddmm_case",
     "SUPPORTED_INDEX_DTYPES",
     "SUPPORTED_VALUE_DTYPES",
     "error_metrics",
@@ -60,6 +69,7 @@ __all__ = [
     "prepare_spmv_csr",
     "SPMM_CSR_ALGORITHMS",
     "PreparedCsrSpmmOpt",
+    "PreparedCsrSpmmOptAlg2",
     "PreparedCsrSpmmRoute",
     "SpmmCsrAlgorithm",
     "SpmmCsrAlgorithmUnavailable",
@@ -67,11 +77,15 @@ __all__ = [
     "flagsparse_spmm_csr_opt",
     "flagsparse_spmm_csr_opt_alg1",
     "flagsparse_spmm_csr_opt_alg1_preprocess",
+    "flagsparse_spmm_csr_opt_alg2",
+    "flagsparse_spmm_csr_opt_alg2_preprocess",
     "flagsparse_spmm_csr_run",
     "list_spmm_csr_algorithms",
     "prepare_spmm_csr_opt",
     "prepare_spmm_csr_opt_alg1",
     "prepare_spmm_csr_opt_alg1_preprocess",
+    "prepare_spmm_csr_opt_alg2",
+    "prepare_spmm_csr_opt_alg2_preprocess",
     "prepare_spmm_csr_route",
     "resolve_spmm_csr_algorithm",
     "SDDMMPrepared",

@@ -215,6 +215,35 @@ def tune_spmv(handle, data, allow_tuning=True):
     return best
 
 
+def tune_spmm(handle, data, n_dense, allow_tuning=True):
+    """Parameters of C = A B (B row-major, n_dense columns) for a CsrHandle:
+    the cached winner of its shape class, or tuned now on this matrix with a
+    B of ones (C scratch)."""
+    device = data.device
+    lengths = handle.indptr[1:] - handle.indptr[:-1] if handle.n_rows else handle.indptr[:0]
+    max_row = int(lengths.max().item()) if handle.n_rows else 0
+    key = shape_class("spmm", data.dtype, handle.indices.dtype, handle.indptr.dtype, n_dense, handle.n_rows,
+                      handle.nnz, max_row, compute_units(device))
+    hit = lookup(key)
+    if hit is not None or not allow_tuning or handle.n_rows == 0 or n_dense == 0:
+        return hit or TuneParams()
+    elem = data.element_size()
+    B = torch.ones(handle.n_cols, n_dense, dtype=data.dtype, device=device)
+    C = torch.empty(handle.n_rows, n_dense, dtype=data.dtype, device=device)
+    cands = candidates("spmm", n_dense, elem)
+    default_lanes = _plan.lanes_for_dense(n_dense, elem)
+    for p in cands:
+        handle.plan(p.spmm_config(p.lanes_n or default_lanes))
+
+    def run(p):
+        ln = p.lanes_n or default_lanes
+        handle.spmm(data, B, out=C, lanes_n=ln, config=p.spmm_config(ln), max_blocks=p.max_blocks(device),
+                    xcd_groups=p.xcd_groups)
+
+    best, _ = tune(key, cands, run)
+    return best
+
+
 
 def time_launch(fn, reps=5):
     """Median device time (ms) of fn() on the current stream after a warm-up."""

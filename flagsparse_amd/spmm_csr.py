@@ -2,7 +2,8 @@
 
 Drop-in for the reference's spmm_csr operator (src/flagsparse/
 sparse_operations/spmm_csr.py): ``flagsparse_spmm_csr`` (:3981-4137), the
-opt / alg1 prepared API (:2720-3253, :4140-4301) and the route API
+opt / alg1 prepared API (:2720-3253, :4140-4301), the opt-alg2 prepared API
+(spmm_csr_opt_alg2.py:39-76, :916-1241) and the route API
 (``SPMM_CSR_ALGORITHMS``, ``prepare_spmm_csr_route``,
 ``flagsparse_spmm_csr_run``; :619-842, :2452-2717).  Validation order,
 exception types and messages follow the reference.  Every route runs the same
@@ -17,7 +18,7 @@ from dataclasses import dataclass
 
 import torch
 
-from . import _plan
+from . import _plan, autotune
 from ._common import (
     OP_CONJ_TRANS,
     OP_NAMES,
@@ -432,6 +433,167 @@ def flagsparse_spmm_csr_opt(data=None, indices=None, indptr=None, B=None, shape=
                             return_time=False, return_meta=False):
     """Compatibility alias for CSR SpMM-opt alg1."""
     return flagsparse_spmm_csr_opt_alg1(data=data, indices=indices, indptr=indptr, B=B, shape=shape,
+                                        prepared=prepared, out=out, return_time=return_time,
+                                        return_meta=return_meta)
+
+
+# ------------------------------------------------------------ opt_alg2 API
+SUPPORTED_SPMM_OPT_ALG2_DTYPES = (torch.float32, torch.float64)
+
+
+class PreparedCsrSpmmOptAlg2:
+    """Cached CSR metadata for repeated opt-alg2 SpMM calls (reference:
+    spmm_csr_opt_alg2.py:39-76).  opt-alg2 is the device-aware route: its
+    launch parameters are the autotuner's winner for the matrix's shape class
+    on this GPU (row-length histogram, dense width, compute units), tuned on
+    the first call of a class and cached.  ``opt_buckets`` describes the row
+    bins of the tuned plan."""
+
+    __slots__ = (
+        "data",
+        "kernel_indices",
+        "kernel_indptr",
+        "shape",
+        "n_rows",
+        "n_cols",
+        "row_lengths",
+        "max_row_nnz",
+        "opt_buckets",
+        "supports_opt",
+        "_handle",
+        "_params",
+    )
+
+    def __init__(self, data, kernel_indices, kernel_indptr, shape, n_rows, n_cols, row_lengths, max_row_nnz,
+                 opt_buckets=None):
+        self.data = data
+        self.kernel_indices = kernel_indices
+        self.kernel_indptr = kernel_indptr
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.n_rows = int(n_rows)
+        self.n_cols = int(n_cols)
+        self.row_lengths = row_lengths
+        self.max_row_nnz = int(max_row_nnz)
+        self.opt_buckets = list(opt_buckets or [])
+        self.supports_opt = data.dtype in SUPPORTED_SPMM_OPT_ALG2_DTYPES
+        self._handle = CsrHandle(kernel_indices, kernel_indptr, self.shape)
+        self._params = {}  # n_dense -> TuneParams
+
+
+def prepare_spmm_csr_opt_alg2(data, indices, indptr, shape):
+    """spmm_csr_opt_alg2.py:916-936."""
+    (data, kernel_indices, kernel_indptr, n_rows, n_cols, row_lengths, max_row_nnz) = _prepare_spmm_csr_matrix(
+        data, indices, indptr, shape
+    )
+    return PreparedCsrSpmmOptAlg2(data, kernel_indices, kernel_indptr, shape, n_rows, n_cols, row_lengths,
+                                  max_row_nnz)
+
+
+def prepare_spmm_csr_opt_alg2_preprocess(data, indices, indptr, shape):
+    return prepare_spmm_csr_opt_alg2(data, indices, indptr, shape)
+
+
+def _validate_spmm_opt_alg2_runtime_inputs(prepared, B, out):
+    """Checks and messages in the order of spmm_csr_opt_alg2.py:943-978."""
+    if not prepared.supports_opt:
+        raise TypeError("flagsparse_spmm_csr_opt_alg2 only supports float32 and float64")
+    if B is None:
+        raise ValueError("B is required")
+    if B.ndim != 2:
+        raise ValueError("B must be a 2D dense tensor")
+    if not B.is_cuda:
+        raise ValueError("B must be a CUDA tensor")
+    if B.device != prepared.data.device:
+        raise ValueError("B must be on the same CUDA device as sparse matrix data")
+    if B.dtype != prepared.data.dtype:
+        raise TypeError("B dtype must match sparse matrix dtype")
+    if B.shape[0] != prepared.n_cols:
+        raise ValueError(f"B.shape[0] must be n_cols={prepared.n_cols}, got {B.shape[0]}")
+    if out is not None:
+        if not out.is_cuda:
+            raise ValueError("out must be a CUDA tensor")
+        if out.device != prepared.data.device:
+            raise ValueError("out must be on the same CUDA device as sparse matrix data")
+        if tuple(out.shape) != (prepared.n_rows, int(B.shape[1])) or out.dtype != prepared.data.dtype:
+            raise ValueError("out shape/dtype must match result")
+
+
+def flagsparse_spmm_csr_opt_alg2(data=None, indices=None, indptr=None, B=None, shape=None, prepared=None,
+                                 out=None, return_time=False, return_meta=False):
+    """CSR SpMM opt-alg2: the autotuned row-bin path for CSR @ dense
+    (argument contract, returns and meta keys of spmm_csr_opt_alg2.py:1136-1215)."""
+    if prepared is not None and not isinstance(prepared, PreparedCsrSpmmOptAlg2):
+        raise TypeError("prepared must be a PreparedCsrSpmmOptAlg2 instance")
+    if prepared is None:
+        if any(arg is None for arg in (data, indices, indptr, shape)):
+            raise ValueError("data, indices, indptr, and shape are required when prepared is not provided")
+        prepared = prepare_spmm_csr_opt_alg2(data, indices, indptr, shape)
+    elif shape is not None:
+        resolved_shape = (int(shape[0]), int(shape[1]))
+        if resolved_shape != prepared.shape:
+            raise ValueError(f"shape {resolved_shape} does not match prepared.shape {prepared.shape}")
+    _validate_spmm_opt_alg2_runtime_inputs(prepared, B, out)
+    B = B.contiguous()
+    n_dense = int(B.shape[1])
+    device = prepared.data.device
+    timer = _Timer(return_time or return_meta)
+    timer.mark()
+    params = prepared._params.get(n_dense)
+    if params is None:
+        params = autotune.tune_spmm(prepared._handle, prepared.data, n_dense)
+        prepared._params[n_dense] = params
+    lanes = params.lanes_n or _plan.lanes_for_dense(n_dense, B.element_size())
+    cfg = params.spmm_config(lanes)
+    plan = prepared._handle.plan(cfg) if prepared.n_rows else None
+    if plan is not None:
+        prepared.opt_buckets = [
+            {"kind": "rows", "slots": int(l), "max_row_nnz": int(u), "rows": int(c)}
+            for l, u, c in zip(plan.config.lanes, plan.config.uppers, plan.bin_counts)
+            if c
+        ]
+        if plan.n_segments:
+            prepared.opt_buckets.append({"kind": "split", "rows": plan.n_long_rows, "parts": plan.n_segments})
+    symbolic_ms = timer.mark()
+    direct_out = out if (out is not None and out.is_contiguous()) else None
+    C = prepared._handle.spmm(prepared.data, B, out=direct_out, lanes_n=lanes, config=cfg,
+                              max_blocks=params.max_blocks(device) if prepared.n_rows else 0,
+                              xcd_groups=params.xcd_groups)
+    compute_ms = timer.mark()
+    op_total_ms = None if compute_ms is None else symbolic_ms + compute_ms
+    if out is not None:
+        if C.data_ptr() != out.data_ptr():
+            out.copy_(C)
+        C = out
+    meta = None
+    if return_meta:
+        props = torch.cuda.get_device_properties(device)
+        meta = {
+            "device_name": props.name,
+            "sm_count": int(props.multi_processor_count),
+            "warp_size": int(getattr(props, "warp_size", 64)),
+            "bucket_hits": {f"{b['kind']}{i}": int(b["rows"]) for i, b in enumerate(prepared.opt_buckets)},
+            "launch_configs": {"unroll": int(params.unroll), "slot_scale": int(params.slot_scale),
+                               "lanes_n": int(lanes), "blocks_per_cu": int(params.blocks_per_cu),
+                               "xcd_groups": int(params.xcd_groups)},
+            "max_row_nnz": prepared.max_row_nnz,
+            "n_dense_cols": n_dense,
+            "symbolic_ms": symbolic_ms,
+            "compute_ms": compute_ms,
+            "op_total_ms": op_total_ms,
+        }
+    if return_time and return_meta:
+        return C, op_total_ms, meta
+    if return_time:
+        return C, op_total_ms
+    if return_meta:
+        return C, meta
+    return C
+
+
+def flagsparse_spmm_csr_opt_alg2_preprocess(data=None, indices=None, indptr=None, B=None, shape=None,
+                                            prepared=None, out=None, return_time=False, return_meta=False):
+    """Compatibility alias for CSR SpMM opt-alg2 (spmm_csr_opt_alg2.py:1218-1241)."""
+    return flagsparse_spmm_csr_opt_alg2(data=data, indices=indices, indptr=indptr, B=B, shape=shape,
                                         prepared=prepared, out=out, return_time=return_time,
                                         return_meta=return_meta)
 

@@ -200,6 +200,42 @@ def test_spmm_opt_matches_oracle(device, dtype, index_dtype):
         fs.flagsparse_spmm_csr_opt(B=torch.randn(K, N, device=device).half(), prepared=p16)
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
+@pytest.mark.parametrize("index_dtype", [torch.int32, torch.int64], ids=["i32", "i64"])
+def test_spmm_opt_alg2_matches_oracle(device, dtype, index_dtype, monkeypatch):
+    # opt-alg2 = the autotuned launch; every candidate must give the oracle's result
+    monkeypatch.setenv("FLAGSPARSE_AMD_TUNE_CACHE", "")
+    cases = [(M, N, K) + _case(M, K, seed=M + K) for M, N, K in MNK]
+    ip, ix, v, Bp = synthetic.power_law_rows(3000, 3000, 2.1, 24, 1024, n_dense=32, seed=5)
+    cases.append((3000, 32, 3000, ip, ix, v))
+    for M, N, K, ip, ix, v in cases:
+        data, idx, ptr = device_csr(ip, ix, v, device, dtype, index_dtype)
+        B = np.random.default_rng(4).standard_normal((K, N)) * 0.125
+        Bt = torch.tensor(B, dtype=dtype, device=device)
+        prepared = fs.prepare_spmm_csr_opt_alg2(data, idx, ptr, (M, K))
+        C, ms, meta = fs.flagsparse_spmm_csr_opt_alg2(B=Bt, prepared=prepared, return_time=True, return_meta=True)
+        ref = oracle.spmm(ip, ix, np_dtype_round(v, dtype), np_dtype_round(B, dtype), (M, K))
+        assert_rel(C, ref, dtype, f"alg2 {M}x{K} n{N}")
+        assert {"device_name", "sm_count", "warp_size", "bucket_hits", "launch_configs", "max_row_nnz",
+                "n_dense_cols", "symbolic_ms", "compute_ms", "op_total_ms"} <= set(meta)
+        assert meta["n_dense_cols"] == N and meta["op_total_ms"] == ms and ms >= 0
+        out = torch.full((M, N), 7.0, dtype=dtype, device=device)
+        got = fs.flagsparse_spmm_csr_opt_alg2_preprocess(data, idx, ptr, Bt, (M, K), out=out)
+        assert got is out and torch.equal(out, C)
+    with pytest.raises(TypeError, match="only supports float32"):
+        p16 = fs.prepare_spmm_csr_opt_alg2(data.half(), idx, ptr, (M, K))
+        fs.flagsparse_spmm_csr_opt_alg2(B=Bt.half(), prepared=p16)
+    with pytest.raises(TypeError, match="PreparedCsrSpmmOptAlg2"):
+        fs.flagsparse_spmm_csr_opt_alg2(B=Bt, prepared=fs.prepare_spmm_csr_opt(data, idx, ptr, (M, K)))
+    with pytest.raises(ValueError, match="does not match prepared.shape"):
+        fs.flagsparse_spmm_csr_opt_alg2(B=Bt, prepared=prepared, shape=(M + 1, K))
+    with pytest.raises(ValueError, match="B.shape\\[0\\] must be n_cols"):
+        fs.flagsparse_spmm_csr_opt_alg2(B=Bt[1:], prepared=prepared)
+    with pytest.raises(ValueError, match="out shape/dtype"):
+        fs.flagsparse_spmm_csr_opt_alg2(B=Bt, prepared=prepared, out=torch.empty(M, N + 1, dtype=dtype,
+                                                                                  device=device))
+
+
 @pytest.mark.parametrize("alg", sorted(fs.SPMM_CSR_ALGORITHMS))
 @pytest.mark.parametrize("op", ["non", "trans"])
 @pytest.mark.parametrize("layout", ["row", "col"])
