@@ -134,6 +134,7 @@ class CsrHandle:
                 y.data_ptr(),
                 n_out,
                 self.nnz,
+                int(x.shape[0]),
                 st,
                 _lib.stream_ptr(x.device),
             ),

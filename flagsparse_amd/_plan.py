@@ -57,6 +57,19 @@ class PlanConfig:
     unroll: int
 
 
+# SpMV variant 2 stages x in LDS: x of at most this many bytes, and only for
+# matrices large enough that every workgroup's one copy of x (a persistent
+# grid of two 1024-thread workgroups per CU) is small beside the nonzeros
+LDSX_MAX_BYTES = 65536
+LDSX_MIN_NNZ = 1 << 20
+
+
+def spmv_default_variant(x_len, elem_bytes, nnz):
+    """SpMV loop used without tuning: 2 (x in LDS) when x fits and the matrix
+    is large, else 0 (software-pipelined)."""
+    return 2 if (0 < x_len * elem_bytes <= LDSX_MAX_BYTES and nnz >= LDSX_MIN_NNZ) else 0
+
+
 def spmv_config(unroll=SPMV_UNROLL, long_threshold=SPMV_LONG_THRESHOLD, seg_len=SPMV_SEG_LEN):
     lanes = (4, 8, 16, 32, 64)
     uppers = [min(l * unroll, long_threshold) for l in lanes[:-1]] + [long_threshold]
@@ -162,7 +175,8 @@ class CsrPlan:
         that many workgroups, each walking a contiguous run of row tiles;
         xcd_groups > 1 cuts each bin's tiles into that many contiguous regions
         served by the workgroups b with b % xcd_groups == region (8 = one
-        region per XCD).  variant selects the SpMV loop (0 pipelined, 1 plain)."""
+        region per XCD).  variant selects the SpMV loop (0 pipelined, 1 plain,
+        2 plain with x staged in LDS)."""
         st = _lib.FspCsrPlan()
         st.n_bins = len(self.bin_counts)
         st.unroll = self.config.unroll

@@ -15,7 +15,8 @@ The tuned parameters are the ones the kernels take from their plan:
                      contiguous run of rows;
 * ``xcd_groups``     8 = cut each row bin into 8 contiguous regions, one per
                      XCD, so neighbouring rows share an L2 (0 = off);
-* ``variant``        SpMV loop: 0 = software-pipelined, 1 = plain.
+* ``variant``        SpMV loop: 0 = software-pipelined, 1 = plain, 2 = plain
+                     with x staged in LDS (x of at most 64 KiB).
 
 A shape class is (kernel, value / index dtypes, dense width, rounded log2 of
 the mean and maximum row length and of the row count, compute units).  The
@@ -76,10 +77,11 @@ def _log2_bucket(v):
     return int(round(math.log2(v))) if v > 0 else -1
 
 
-def shape_class(kind, value_dtype, index_dtype, indptr_dtype, n_dense, n_rows, nnz, max_row, cus):
-    """Key of a shape class (a string, stable across processes)."""
+def shape_class(kind, value_dtype, index_dtype, indptr_dtype, n_dense, n_rows, nnz, max_row, cus, extra=""):
+    """Key of a shape class (a string, stable across processes); `extra`
+    tags classes whose candidate sets differ (e.g. x fits in LDS)."""
     mean = nnz / n_rows if n_rows else 0.0
-    return "|".join([
+    parts = [
         kind,
         str(value_dtype).replace("torch.", ""),
         str(index_dtype).replace("torch.", ""),
@@ -89,11 +91,16 @@ def shape_class(kind, value_dtype, index_dtype, indptr_dtype, n_dense, n_rows, n
         f"max{_log2_bucket(max_row)}",
         f"rows{_log2_bucket(n_rows)}",
         f"cu{int(cus)}",
-    ])
+    ]
+    if extra:
+        parts.append(extra)
+    return "|".join(parts)
 
 
-def candidates(kind, n_dense=1, elem_bytes=4):
-    """Candidate parameter sets of a kernel family (small, fixed grids)."""
+def candidates(kind, n_dense=1, elem_bytes=4, x_len=0):
+    """Candidate parameter sets of a kernel family (small, fixed grids).
+    For SpMV, x_len > 0 with x_len * elem_bytes <= 64 KiB adds the loop with
+    x staged in LDS (variant 2: 1024-thread workgroups, 1 or 2 per CU)."""
     out = []
     if kind == "spmv":
         for variant in (0, 1):
@@ -101,6 +108,11 @@ def candidates(kind, n_dense=1, elem_bytes=4):
                 for bpc in (0, 4, 8):
                     for xg in (0, 8):
                         out.append(TuneParams(unroll=unroll, blocks_per_cu=bpc, xcd_groups=xg, variant=variant))
+        if 0 < x_len * elem_bytes <= _plan.LDSX_MAX_BYTES:
+            for unroll in (2, 4, 8):
+                for bpc in (1, 2):
+                    for xg in (0, 8):
+                        out.append(TuneParams(unroll=unroll, blocks_per_cu=bpc, xcd_groups=xg, variant=2))
         return out
     g = _plan.lanes_for_dense(n_dense, elem_bytes)
     lanes = [g] + ([g // 2] if g >= 8 else [])
@@ -196,14 +208,15 @@ def tune_spmv(handle, data, allow_tuning=True):
     device = data.device
     lengths = handle.indptr[1:] - handle.indptr[:-1] if handle.n_rows else handle.indptr[:0]
     max_row = int(lengths.max().item()) if handle.n_rows else 0
+    x_lds = 0 < handle.n_cols * data.element_size() <= _plan.LDSX_MAX_BYTES
     key = shape_class("spmv", data.dtype, handle.indices.dtype, handle.indptr.dtype, 1, handle.n_rows, handle.nnz,
-                      max_row, compute_units(device))
+                      max_row, compute_units(device), extra="xlds" if x_lds else "")
     hit = lookup(key)
     if hit is not None or not allow_tuning:
         return hit or TuneParams()
     x = torch.ones(handle.n_cols, dtype=data.dtype, device=device)
     y = torch.empty(handle.n_rows, dtype=data.dtype, device=device)
-    cands = candidates("spmv")
+    cands = candidates("spmv", 1, data.element_size(), handle.n_cols)
     for p in cands:  # plans (host syncs) before any timing
         handle.plan(p.spmv_config())
 

@@ -9,6 +9,8 @@
 // one row earlier).  Rows longer than the last bin are cut into segments; one wave
 // reduces each segment into a partial and a second pass adds a row's partials
 // in segment order (no atomics: the result is deterministic for a plan).
+#include <atomic>
+
 #include "fsp_device.hpp"
 #include "fsp_host.hpp"
 #include "fsp_plan.hpp"
@@ -170,6 +172,94 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmv_simple_kernel(
   }
 }
 
+// x staged in LDS (plan.variant = 2), for x of at most kLdsxBytes: each
+// 1024-thread workgroup copies x into LDS once and then runs the plain loop
+// above with its x gathers served by LDS.  Through the vector L1 every
+// gathered element is a separate cache-line lookup, and for short rows those
+// lookups, not HBM, bound the loop (the x reads are as many as the
+// nonzeros); from LDS the index / value stream runs at the device copy rate
+// (measured on MI355X).  The grid is persistent (plan.max_blocks, default
+// two workgroups per CU) so x is loaded once per workgroup.  Per lane the
+// arithmetic is that of spmv_simple_kernel: results are bitwise equal.
+constexpr int kLdsxWaves = 16;
+constexpr int kLdsxThreads = 64 * kLdsxWaves;
+constexpr int64_t kLdsxBytes = 65536;
+
+template <typename T, typename IT, typename PT, typename QT, int UNROLL, bool PERM>
+__global__ __launch_bounds__(kLdsxThreads) void spmv_ldsx_kernel(
+    const PT* __restrict__ indptr, const IT* __restrict__ indices, const T* __restrict__ data,
+    const QT* __restrict__ perm, const T* __restrict__ x, T* __restrict__ y,
+    typename spmv_acc_of<T>::type* __restrict__ partials, launch_table tab, int64_t x_len) {
+  using A = typename spmv_acc_of<T>::type;
+  extern __shared__ __align__(16) unsigned char fsp_ldsx_mem[];
+  T* xs = reinterpret_cast<T*>(fsp_ldsx_mem);
+  for (int64_t c = threadIdx.x; c < x_len; c += kLdsxThreads) xs[c] = x[c];
+  __syncthreads();
+  const int k = find_bin(tab, blockIdx.x);
+  const launch_bin& lb = tab.b[k];
+  const int ll = lb.lanes_log2;
+  const int L = 1 << ll;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int sub = lane >> ll;
+  const int l = lane & (L - 1);
+  const int64_t per_wave = 64 >> ll;
+  const int64_t per_block = per_wave * kLdsxWaves;
+  const tile_range tr = bin_tiles(lb, per_block);
+  for (int64_t tile = tr.first; tile < tr.end; tile += tr.step) {
+    const int64_t rr = tile * per_block + wave * per_wave + sub;
+    const bool active = rr < lb.count;
+    A acc = A(0);
+    work_item w{0, 0, 0, 0};
+    if (active) {
+      w = load_item(tab, lb, indptr, rr);
+      int64_t p = w.begin + l;
+      for (; p + (UNROLL - 1) * L < w.end; p += UNROLL * L) {
+        int64_t c[UNROLL];
+        T a[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+          const int64_t q = p + u * L;
+          c[u] = static_cast<int64_t>(indices[q]);
+          a[u] = data[PERM ? static_cast<int64_t>(perm[q]) : q];
+        }
+        T xv[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) xv[u] = xs[c[u]];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) acc += static_cast<A>(a[u]) * static_cast<A>(xv[u]);
+      }
+      for (; p < w.end; p += L) {
+        const int64_t q = PERM ? static_cast<int64_t>(perm[p]) : p;
+        acc += static_cast<A>(data[q]) * static_cast<A>(xs[static_cast<int64_t>(indices[p])]);
+      }
+    }
+    acc = group_sum(acc, L);
+    if (active && l == 0) {
+      if (lb.is_segment) {
+        partials[w.out] = acc;
+      } else {
+        y[w.out] = static_cast<T>(acc);
+      }
+    }
+  }
+}
+
+// Compute units of the current device (cached per device).
+static int device_cu_count() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 0;
+  if (dev < 64) {
+    const int v = cache[dev].load(std::memory_order_relaxed);
+    if (v > 0) return v;
+  }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (dev < 64) cache[dev].store(cus, std::memory_order_relaxed);
+  return cus;
+}
+
 template <typename T>
 __global__ __launch_bounds__(FSP_WG_THREADS) void spmv_zero_kernel(T* __restrict__ y, int64_t n) {
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * FSP_WG_THREADS + threadIdx.x; i < n;
@@ -194,12 +284,24 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmv_long_reduce_kernel(
 
 template <typename T, typename IT, typename PT, typename QT, bool PERM>
 static int launch_spmv(const void* indptr, const void* indices, const void* data, const void* perm,
-                       const void* x, void* y, const fsp_csr_plan* plan, hipStream_t s) {
+                       const void* x, void* y, int64_t x_len, const fsp_csr_plan* plan, hipStream_t s) {
   using A = typename spmv_acc_of<T>::type;
   launch_table tab;
+  const bool ldsx = plan->variant == 2;
+  const int64_t x_bytes = x_len * static_cast<int64_t>(sizeof(T));
+  if (ldsx && (x_len < 1 || x_bytes > kLdsxBytes)) {
+    return fail(FSP_ERR_ARG, "fsp_spmv_csr: variant 2 stages x in LDS and needs 1 <= x_len * %d <= %lld bytes",
+                (int)sizeof(T), (long long)kLdsxBytes);
+  }
+  fsp_csr_plan lplan = *plan;
+  if (ldsx && lplan.max_blocks == 0) {
+    const int cus = device_cu_count();
+    lplan.max_blocks = 2 * static_cast<int64_t>(cus > 0 ? cus : 256);
+  }
+  const int64_t waves = ldsx ? kLdsxWaves : FSP_WAVES;
   const int64_t blocks = build_launch_table(
-      plan, /*seg_lanes_log2=*/6, /*max_lanes_log2=*/6,
-      [](int ll, int) { return int64_t(FSP_WAVES) * (int64_t(64) >> ll); }, &tab);
+      &lplan, /*seg_lanes_log2=*/6, /*max_lanes_log2=*/6,
+      [waves](int ll, int) { return waves * (int64_t(64) >> ll); }, &tab);
   if (plan->n_segments > 0 &&
       (plan->partials_dev == nullptr || plan->partials_bytes < plan->n_segments * (int64_t)sizeof(A))) {
     return fail(FSP_ERR_WORKSPACE, "fsp_spmv_csr: partials need %lld bytes",
@@ -209,7 +311,13 @@ static int launch_spmv(const void* indptr, const void* indices, const void* data
   if (blocks > 0) {
     const int unroll = plan->unroll;
 #define FSP_SPMV_LAUNCH(U)                                                                          \
-  if (plan->variant == 1) {                                                                          \
+  if (ldsx) {                                                                                        \
+    hipLaunchKernelGGL((spmv_ldsx_kernel<T, IT, PT, QT, U, PERM>), dim3(blocks), dim3(kLdsxThreads),    \
+                       static_cast<size_t>(x_bytes), s, static_cast<const PT*>(indptr),              \
+                       static_cast<const IT*>(indices), static_cast<const T*>(data),                  \
+                       static_cast<const QT*>(perm), static_cast<const T*>(x), static_cast<T*>(y),     \
+                       partials, tab, x_len);                                                        \
+  } else if (plan->variant == 1) {                                                                   \
     hipLaunchKernelGGL((spmv_simple_kernel<T, IT, PT, QT, U, PERM>), dim3(blocks), dim3(FSP_WG_THREADS), 0, \
                        s, static_cast<const PT*>(indptr), static_cast<const IT*>(indices),          \
                        static_cast<const T*>(data), static_cast<const QT*>(perm),                    \
@@ -248,8 +356,8 @@ using namespace fsp;
 extern "C" int fsp_spmv_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
                             const void* indptr_dev, const void* indices_dev, const void* data_dev,
                             int32_t perm_type, const void* perm_dev, const void* x_dev, void* y_dev,
-                            int64_t n_rows, int64_t nnz, const fsp_csr_plan* plan, void* stream) {
-  if (n_rows < 0 || nnz < 0) return fail(FSP_ERR_ARG, "fsp_spmv_csr: negative n_rows or nnz");
+                            int64_t n_rows, int64_t nnz, int64_t x_len, const fsp_csr_plan* plan, void* stream) {
+  if (n_rows < 0 || nnz < 0 || x_len < 0) return fail(FSP_ERR_ARG, "fsp_spmv_csr: negative n_rows, nnz or x_len");
   int st = check_plan(plan, n_rows);
   if (st) return st;
   if (n_rows == 0) return FSP_OK;
@@ -273,11 +381,11 @@ extern "C" int fsp_spmv_csr(int32_t value_type, int32_t index_type, int32_t indp
       FSP_DISPATCH_INDEX(indptr_type, PT, {
         if (!use_perm) {
           return launch_spmv<T, IT, PT, int32_t, false>(indptr_dev, indices_dev, data_dev, nullptr, x_dev,
-                                                        y_dev, plan, s);
+                                                        y_dev, x_len, plan, s);
         }
         FSP_DISPATCH_INDEX(perm_type, QT, {
           return launch_spmv<T, IT, PT, QT, true>(indptr_dev, indices_dev, data_dev, perm_dev, x_dev, y_dev,
-                                                  plan, s);
+                                                  x_len, plan, s);
         });
       });
     });

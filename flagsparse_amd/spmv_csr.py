@@ -281,7 +281,8 @@ def _hip_spmv_csr_impl_prepared(prepared, x, out=None):
     plan = prepared.plan()
     acc = torch.float64 if dtype in (torch.float32, torch.float64) else torch.float32
     partials = plan.partials(1, acc, device)
-    st = plan.struct(partials)
+    nnz = int(prepared.kernel_indices.numel())
+    st = plan.struct(partials, variant=_plan.spmv_default_variant(int(x.shape[0]), x.element_size(), nnz))
     _lib.check(
         _lib.load().fsp_spmv_csr(
             _lib.value_code(dtype),
@@ -295,7 +296,8 @@ def _hip_spmv_csr_impl_prepared(prepared, x, out=None):
             _lib.ptr(x),
             y.data_ptr(),
             prepared.n_rows,
-            int(prepared.kernel_indices.numel()),
+            nnz,
+            int(x.shape[0]),
             st,
             _lib.stream_ptr(device),
         ),

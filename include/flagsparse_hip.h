@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSP_ABI_VERSION 6
+#define FSP_ABI_VERSION 7
 #define FSP_MAX_BINS 8
 
 typedef enum fsp_value_type {
@@ -83,7 +83,11 @@ typedef struct fsp_csr_plan {
                                      on MI355X: neighbouring rows share an L2) */
   int64_t variant;                /* SpMV loop: 0 = software-pipelined (next trip's
                                      indices load under this trip's x gathers),
-                                     1 = plain loop; other kernels ignore it */
+                                     1 = plain loop, 2 = plain loop with x staged
+                                     in LDS by 1024-thread workgroups (x_len *
+                                     element size <= 64 KiB; persistent grid,
+                                     max_blocks 0 = two per CU); other kernels
+                                     ignore it */
 } fsp_csr_plan;
 
 /* ---- library ------------------------------------------------------------ */
@@ -134,12 +138,13 @@ int fsp_gather(int32_t elem_bytes, int32_t perm_type, const void* perm_dev,
  * kernels of the use_opt path (spmv_csr.py:247-368, 423-482).
  * Accumulation: f64 for f32/f64 values (the reference promotes f32 SpMV to
  * f64, spmv_csr.py:138-143), f32 for f16/bf16.  y has value_type.  nnz = 0
- * writes y = 0 (indices / data / x may then be NULL). */
+ * writes y = 0 (indices / data / x may then be NULL).  x_len = elements of x
+ * (columns of the operator; every column index is below it). */
 int fsp_spmv_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
                  const void* indptr_dev, const void* indices_dev,
                  const void* data_dev, int32_t perm_type, const void* perm_dev,
                  const void* x_dev, void* y_dev, int64_t n_rows, int64_t nnz,
-                 const fsp_csr_plan* plan, void* stream);
+                 int64_t x_len, const fsp_csr_plan* plan, void* stream);
 
 /* ---- SpMM ----------------------------------------------------------------
  * C[n_rows, n_dense] = A B with B row-major (ldb >= n_dense) and C row-major

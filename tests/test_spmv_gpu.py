@@ -253,3 +253,54 @@ def test_spmv_loop_variants_and_grids_bitwise_equal(device, variant, max_blocks,
         got = h.spmv(data, xt, config=cfg, max_blocks=max_blocks, xcd_groups=xcd, variant=variant)
         assert torch.equal(got, ref)
         assert_rel(got, oracle.spmv(ip_, ix_, v_, x, shape), torch.float32, "spmv variant")
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.bfloat16, torch.float16],
+                         ids=["f32", "f64", "bf16", "f16"])
+@pytest.mark.parametrize("unroll", [2, 4, 8])
+def test_spmv_x_in_lds_variant_bitwise_equal_to_plain_loop(device, dtype, unroll):
+    """Variant 2 (x staged in LDS) runs the plain loop's arithmetic per lane:
+    bitwise equal to variant 1 on every grid, for CSR and CSC (trans) launches,
+    with long-row segments, and within tolerance of the oracle."""
+    from flagsparse_amd import _plan
+    from flagsparse_amd._matrix import CsrHandle
+
+    ip, ix, v = long_row_matrix(5000, 3000, long_rows=(7, 4000), long_len=2900, short_len=9, seed=21)
+    u_ip, u_ix, u_v, _ = synthetic.uniform_rows(4096, 8192 if dtype == torch.float64 else 16384, 16, seed=0)
+    cases = [(ip, ix, v, (5000, 3000)), (u_ip, u_ix, u_v, (4096, 8192 if dtype == torch.float64 else 16384))]
+    cfg = _plan.spmv_config(unroll=unroll)
+    for ip_, ix_, v_, shape in cases:
+        data, idx, ptr = device_csr(ip_, ix_, v_, device, dtype)
+        h = CsrHandle(idx, ptr, shape)
+        for trans in (False, True):
+            n_x = shape[0] if trans else shape[1]
+            if n_x * data.element_size() > 65536:
+                continue
+            x = np.random.default_rng(3).uniform(-1, 1, n_x)
+            xt = synthetic.dense(x, device, dtype)
+            ref = h.spmv(data, xt, transpose=trans, config=cfg, variant=1)
+            for max_blocks, xcd in ((0, 0), (256, 0), (37, 8), (0, 8)):
+                got = h.spmv(data, xt, transpose=trans, config=cfg, max_blocks=max_blocks, xcd_groups=xcd,
+                             variant=2)
+                assert torch.equal(got, ref), (shape, trans, max_blocks, xcd)
+            gold = oracle.spmv(ip_, ix_, np_dtype_round(v_, dtype), np_dtype_round(x, dtype), shape,
+                               "trans" if trans else "non")
+            assert_rel(ref, gold, dtype, f"ldsx {shape} trans={trans}")
+
+
+def test_spmv_x_in_lds_rejects_large_x_and_default_route(device):
+    from flagsparse_amd._matrix import CsrHandle
+
+    ip, ix, v = synthetic.random_csr(64, 20000, 0.01, seed=4)
+    data, idx, ptr = device_csr(ip, ix, v, device)
+    h = CsrHandle(idx, ptr, (64, 20000))
+    with pytest.raises(RuntimeError, match="LDS"):
+        h.spmv(data, torch.ones(20000, device=device), variant=2)
+    # the public entry point takes the LDS loop for a large matrix with a small x
+    ip, ix, v, x = synthetic.uniform_rows(65536, 16384, 16, seed=0)
+    data, idx, ptr = device_csr(ip, ix, v, device)
+    from flagsparse_amd import _plan
+
+    assert _plan.spmv_default_variant(16384, 4, int(ix.size)) == 2
+    y = fs.flagsparse_spmv_csr(data, idx, ptr, synthetic.dense(x, device), (65536, 16384))
+    assert_rel(y, oracle.spmv(ip, ix, v, x, (65536, 16384)), torch.float32, "default ldsx")
