@@ -176,7 +176,7 @@ class CsrPlan:
         xcd_groups > 1 cuts each bin's tiles into that many contiguous regions
         served by the workgroups b with b % xcd_groups == region (8 = one
         region per XCD).  variant selects the SpMV loop (0 pipelined, 1 plain,
-        2 plain with x staged in LDS)."""
+        2 plain and 3 pipelined with x staged in LDS)."""
         st = _lib.FspCsrPlan()
         st.n_bins = len(self.bin_counts)
         st.unroll = self.config.unroll

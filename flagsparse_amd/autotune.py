@@ -16,7 +16,8 @@ The tuned parameters are the ones the kernels take from their plan:
 * ``xcd_groups``     8 = cut each row bin into 8 contiguous regions, one per
                      XCD, so neighbouring rows share an L2 (0 = off);
 * ``variant``        SpMV loop: 0 = software-pipelined, 1 = plain, 2 = plain
-                     with x staged in LDS (x of at most 64 KiB).
+                     and 3 = pipelined with x staged in LDS (x of at most
+                     64 KiB).
 
 A shape class is (kernel, value / index dtypes, dense width, rounded log2 of
 the mean and maximum row length and of the row count, compute units).  The
@@ -99,8 +100,8 @@ def shape_class(kind, value_dtype, index_dtype, indptr_dtype, n_dense, n_rows, n
 
 def candidates(kind, n_dense=1, elem_bytes=4, x_len=0):
     """Candidate parameter sets of a kernel family (small, fixed grids).
-    For SpMV, x_len > 0 with x_len * elem_bytes <= 64 KiB adds the loop with
-    x staged in LDS (variant 2: 1024-thread workgroups, 1 or 2 per CU)."""
+    For SpMV, x_len > 0 with x_len * elem_bytes <= 64 KiB adds the loops with
+    x staged in LDS (variants 2 and 3: 1024-thread workgroups, 1 or 2 per CU)."""
     out = []
     if kind == "spmv":
         for variant in (0, 1):
@@ -109,10 +110,11 @@ def candidates(kind, n_dense=1, elem_bytes=4, x_len=0):
                     for xg in (0, 8):
                         out.append(TuneParams(unroll=unroll, blocks_per_cu=bpc, xcd_groups=xg, variant=variant))
         if 0 < x_len * elem_bytes <= _plan.LDSX_MAX_BYTES:
-            for unroll in (2, 4, 8):
-                for bpc in (1, 2):
-                    for xg in (0, 8):
-                        out.append(TuneParams(unroll=unroll, blocks_per_cu=bpc, xcd_groups=xg, variant=2))
+            for variant in (2, 3):
+                for unroll in (2, 4, 8):
+                    for bpc in (1, 2):
+                        for xg in (0, 8):
+                            out.append(TuneParams(unroll=unroll, blocks_per_cu=bpc, xcd_groups=xg, variant=variant))
         return out
     g = _plan.lanes_for_dense(n_dense, elem_bytes)
     lanes = [g] + ([g // 2] if g >= 8 else [])

@@ -27,6 +27,11 @@ template <typename T> struct spmv_acc_of { using type = float; };
 template <> struct spmv_acc_of<float> { using type = double; };
 template <> struct spmv_acc_of<double> { using type = double; };
 
+// One rounding per term: c + a * b as a fused multiply-add, written out so
+// every loop variant accumulates identically whatever the contraction mode.
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename A, typename T> __device__ __forceinline__ A to_acc(T v) {
   return static_cast<A>(v);
 }

@@ -17,12 +17,40 @@
 
 namespace fsp {
 
-template <typename T, typename IT, typename PT, typename QT, int UNROLL, bool PERM>
-__global__ __launch_bounds__(FSP_WG_THREADS) void spmv_bins_kernel(
+// x staged in LDS (plan.variant 2 and 3), for x of at most kLdsxBytes: each
+// 1024-thread workgroup copies x into LDS once and the loop then gathers x
+// from LDS.  Through the vector L1 every gathered element is a separate
+// cache-line lookup, and for short rows those lookups, not HBM, bound the
+// loop (the x reads are as many as the nonzeros); from LDS the index / value
+// stream runs near the device copy rate (measured on MI355X).  The grid is
+// persistent (plan.max_blocks, default two workgroups per CU) so x is loaded
+// once per workgroup.  Per lane the arithmetic is unchanged: results are
+// bitwise equal to the loops that read x from global memory.
+constexpr int kLdsxWaves = 16;
+constexpr int kLdsxThreads = 64 * kLdsxWaves;
+constexpr int64_t kLdsxBytes = 65536;
+
+// Pointer to x for the loop: LDS (after the workgroup's copy) or global.
+template <typename T, bool LDSX>
+__device__ __forceinline__ const T* stage_x(const T* __restrict__ x, int64_t x_len) {
+  if constexpr (LDSX) {
+    extern __shared__ __align__(16) unsigned char fsp_ldsx_mem[];
+    T* xs = reinterpret_cast<T*>(fsp_ldsx_mem);
+    for (int64_t c = threadIdx.x; c < x_len; c += blockDim.x) xs[c] = x[c];
+    __syncthreads();
+    return xs;
+  } else {
+    return x;
+  }
+}
+
+template <typename T, typename IT, typename PT, typename QT, int UNROLL, bool PERM, int WAVES, bool LDSX>
+__global__ __launch_bounds__(64 * WAVES) void spmv_bins_kernel(
     const PT* __restrict__ indptr, const IT* __restrict__ indices, const T* __restrict__ data,
-    const QT* __restrict__ perm, const T* __restrict__ x, T* __restrict__ y,
-    typename spmv_acc_of<T>::type* __restrict__ partials, launch_table tab) {
+    const QT* __restrict__ perm, const T* __restrict__ x_in, T* __restrict__ y,
+    typename spmv_acc_of<T>::type* __restrict__ partials, launch_table tab, int64_t x_len) {
   using A = typename spmv_acc_of<T>::type;
+  const T* __restrict__ x = stage_x<T, LDSX>(x_in, x_len);
   const int k = find_bin(tab, blockIdx.x);
   const launch_bin& lb = tab.b[k];
   const int ll = lb.lanes_log2;
@@ -32,7 +60,7 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmv_bins_kernel(
   const int sub = lane >> ll;
   const int l = lane & (L - 1);
   const int64_t per_wave = 64 >> ll;
-  const int64_t per_block = per_wave * FSP_WAVES;
+  const int64_t per_block = per_wave * WAVES;
   const tile_range tr = bin_tiles(lb, per_block);
   const int64_t rr_step = tr.step * per_block;
   const int64_t last_item = lb.count - 1;  // bins in the table are non-empty
@@ -88,7 +116,8 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmv_bins_kernel(
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
         const bool ok = p + static_cast<int64_t>(u) * L < end;
-        acc += ok ? static_cast<A>(a[u]) * static_cast<A>(xv[u]) : A(0);
+        const A t = fmadd(static_cast<A>(a[u]), static_cast<A>(xv[u]), acc);
+        acc = ok ? t : acc;
       }
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
@@ -116,12 +145,13 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmv_bins_kernel(
 // and then gathers x, with no prefetch across trips.  For short uniform rows
 // under a persistent grid it streams the index / value arrays faster than the
 // pipelined loop (measured on MI355X); the autotuner picks between the two.
-template <typename T, typename IT, typename PT, typename QT, int UNROLL, bool PERM>
-__global__ __launch_bounds__(FSP_WG_THREADS) void spmv_simple_kernel(
+template <typename T, typename IT, typename PT, typename QT, int UNROLL, bool PERM, int WAVES, bool LDSX>
+__global__ __launch_bounds__(64 * WAVES) void spmv_simple_kernel(
     const PT* __restrict__ indptr, const IT* __restrict__ indices, const T* __restrict__ data,
-    const QT* __restrict__ perm, const T* __restrict__ x, T* __restrict__ y,
-    typename spmv_acc_of<T>::type* __restrict__ partials, launch_table tab) {
+    const QT* __restrict__ perm, const T* __restrict__ x_in, T* __restrict__ y,
+    typename spmv_acc_of<T>::type* __restrict__ partials, launch_table tab, int64_t x_len) {
   using A = typename spmv_acc_of<T>::type;
+  const T* __restrict__ x = stage_x<T, LDSX>(x_in, x_len);
   const int k = find_bin(tab, blockIdx.x);
   const launch_bin& lb = tab.b[k];
   const int ll = lb.lanes_log2;
@@ -131,7 +161,7 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmv_simple_kernel(
   const int sub = lane >> ll;
   const int l = lane & (L - 1);
   const int64_t per_wave = 64 >> ll;
-  const int64_t per_block = per_wave * FSP_WAVES;
+  const int64_t per_block = per_wave * WAVES;
   const tile_range tr = bin_tiles(lb, per_block);
   for (int64_t tile = tr.first; tile < tr.end; tile += tr.step) {
     const int64_t rr = tile * per_block + wave * per_wave + sub;
@@ -154,84 +184,11 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmv_simple_kernel(
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) xv[u] = x[c[u]];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) acc += static_cast<A>(a[u]) * static_cast<A>(xv[u]);
+        for (int u = 0; u < UNROLL; ++u) acc = fmadd(static_cast<A>(a[u]), static_cast<A>(xv[u]), acc);
       }
       for (; p < w.end; p += L) {
         const int64_t q = PERM ? static_cast<int64_t>(perm[p]) : p;
-        acc += static_cast<A>(data[q]) * static_cast<A>(x[static_cast<int64_t>(indices[p])]);
-      }
-    }
-    acc = group_sum(acc, L);
-    if (active && l == 0) {
-      if (lb.is_segment) {
-        partials[w.out] = acc;
-      } else {
-        y[w.out] = static_cast<T>(acc);
-      }
-    }
-  }
-}
-
-// x staged in LDS (plan.variant = 2), for x of at most kLdsxBytes: each
-// 1024-thread workgroup copies x into LDS once and then runs the plain loop
-// above with its x gathers served by LDS.  Through the vector L1 every
-// gathered element is a separate cache-line lookup, and for short rows those
-// lookups, not HBM, bound the loop (the x reads are as many as the
-// nonzeros); from LDS the index / value stream runs at the device copy rate
-// (measured on MI355X).  The grid is persistent (plan.max_blocks, default
-// two workgroups per CU) so x is loaded once per workgroup.  Per lane the
-// arithmetic is that of spmv_simple_kernel: results are bitwise equal.
-constexpr int kLdsxWaves = 16;
-constexpr int kLdsxThreads = 64 * kLdsxWaves;
-constexpr int64_t kLdsxBytes = 65536;
-
-template <typename T, typename IT, typename PT, typename QT, int UNROLL, bool PERM>
-__global__ __launch_bounds__(kLdsxThreads) void spmv_ldsx_kernel(
-    const PT* __restrict__ indptr, const IT* __restrict__ indices, const T* __restrict__ data,
-    const QT* __restrict__ perm, const T* __restrict__ x, T* __restrict__ y,
-    typename spmv_acc_of<T>::type* __restrict__ partials, launch_table tab, int64_t x_len) {
-  using A = typename spmv_acc_of<T>::type;
-  extern __shared__ __align__(16) unsigned char fsp_ldsx_mem[];
-  T* xs = reinterpret_cast<T*>(fsp_ldsx_mem);
-  for (int64_t c = threadIdx.x; c < x_len; c += kLdsxThreads) xs[c] = x[c];
-  __syncthreads();
-  const int k = find_bin(tab, blockIdx.x);
-  const launch_bin& lb = tab.b[k];
-  const int ll = lb.lanes_log2;
-  const int L = 1 << ll;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int sub = lane >> ll;
-  const int l = lane & (L - 1);
-  const int64_t per_wave = 64 >> ll;
-  const int64_t per_block = per_wave * kLdsxWaves;
-  const tile_range tr = bin_tiles(lb, per_block);
-  for (int64_t tile = tr.first; tile < tr.end; tile += tr.step) {
-    const int64_t rr = tile * per_block + wave * per_wave + sub;
-    const bool active = rr < lb.count;
-    A acc = A(0);
-    work_item w{0, 0, 0, 0};
-    if (active) {
-      w = load_item(tab, lb, indptr, rr);
-      int64_t p = w.begin + l;
-      for (; p + (UNROLL - 1) * L < w.end; p += UNROLL * L) {
-        int64_t c[UNROLL];
-        T a[UNROLL];
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
-          const int64_t q = p + u * L;
-          c[u] = static_cast<int64_t>(indices[q]);
-          a[u] = data[PERM ? static_cast<int64_t>(perm[q]) : q];
-        }
-        T xv[UNROLL];
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) xv[u] = xs[c[u]];
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) acc += static_cast<A>(a[u]) * static_cast<A>(xv[u]);
-      }
-      for (; p < w.end; p += L) {
-        const int64_t q = PERM ? static_cast<int64_t>(perm[p]) : p;
-        acc += static_cast<A>(data[q]) * static_cast<A>(xs[static_cast<int64_t>(indices[p])]);
+        acc = fmadd(static_cast<A>(data[q]), static_cast<A>(x[static_cast<int64_t>(indices[p])]), acc);
       }
     }
     acc = group_sum(acc, L);
@@ -287,10 +244,10 @@ static int launch_spmv(const void* indptr, const void* indices, const void* data
                        const void* x, void* y, int64_t x_len, const fsp_csr_plan* plan, hipStream_t s) {
   using A = typename spmv_acc_of<T>::type;
   launch_table tab;
-  const bool ldsx = plan->variant == 2;
+  const bool ldsx = plan->variant == 2 || plan->variant == 3;
   const int64_t x_bytes = x_len * static_cast<int64_t>(sizeof(T));
   if (ldsx && (x_len < 1 || x_bytes > kLdsxBytes)) {
-    return fail(FSP_ERR_ARG, "fsp_spmv_csr: variant 2 stages x in LDS and needs 1 <= x_len * %d <= %lld bytes",
+    return fail(FSP_ERR_ARG, "fsp_spmv_csr: variants 2 and 3 stage x in LDS and need 1 <= x_len * %d <= %lld bytes",
                 (int)sizeof(T), (long long)kLdsxBytes);
   }
   fsp_csr_plan lplan = *plan;
@@ -310,23 +267,18 @@ static int launch_spmv(const void* indptr, const void* indices, const void* data
   A* partials = static_cast<A*>(plan->partials_dev);
   if (blocks > 0) {
     const int unroll = plan->unroll;
-#define FSP_SPMV_LAUNCH(U)                                                                          \
-  if (ldsx) {                                                                                        \
-    hipLaunchKernelGGL((spmv_ldsx_kernel<T, IT, PT, QT, U, PERM>), dim3(blocks), dim3(kLdsxThreads),    \
-                       static_cast<size_t>(x_bytes), s, static_cast<const PT*>(indptr),              \
-                       static_cast<const IT*>(indices), static_cast<const T*>(data),                  \
-                       static_cast<const QT*>(perm), static_cast<const T*>(x), static_cast<T*>(y),     \
-                       partials, tab, x_len);                                                        \
-  } else if (plan->variant == 1) {                                                                   \
-    hipLaunchKernelGGL((spmv_simple_kernel<T, IT, PT, QT, U, PERM>), dim3(blocks), dim3(FSP_WG_THREADS), 0, \
-                       s, static_cast<const PT*>(indptr), static_cast<const IT*>(indices),          \
-                       static_cast<const T*>(data), static_cast<const QT*>(perm),                    \
-                       static_cast<const T*>(x), static_cast<T*>(y), partials, tab);                 \
-  } else {                                                                                           \
-    hipLaunchKernelGGL((spmv_bins_kernel<T, IT, PT, QT, U, PERM>), dim3(blocks), dim3(FSP_WG_THREADS), 0, s, \
-                       static_cast<const PT*>(indptr), static_cast<const IT*>(indices),              \
-                       static_cast<const T*>(data), static_cast<const QT*>(perm),                    \
-                       static_cast<const T*>(x), static_cast<T*>(y), partials, tab);                 \
+#define FSP_SPMV_KERNEL(NAME, U, W, LX)                                                              \
+  hipLaunchKernelGGL((NAME<T, IT, PT, QT, U, PERM, W, LX>), dim3(blocks), dim3(64 * (W)),                \
+                     LX ? static_cast<size_t>(x_bytes) : 0, s, static_cast<const PT*>(indptr),          \
+                     static_cast<const IT*>(indices), static_cast<const T*>(data),                        \
+                     static_cast<const QT*>(perm), static_cast<const T*>(x), static_cast<T*>(y), partials, \
+                     tab, x_len)
+#define FSP_SPMV_LAUNCH(U)                                                 \
+  switch (plan->variant) {                                                 \
+    case 1: FSP_SPMV_KERNEL(spmv_simple_kernel, U, FSP_WAVES, false); break; \
+    case 2: FSP_SPMV_KERNEL(spmv_simple_kernel, U, kLdsxWaves, true); break; \
+    case 3: FSP_SPMV_KERNEL(spmv_bins_kernel, U, kLdsxWaves, true); break;   \
+    default: FSP_SPMV_KERNEL(spmv_bins_kernel, U, FSP_WAVES, false); break;  \
   }
     if (unroll <= 2) {
       FSP_SPMV_LAUNCH(2);
@@ -336,6 +288,7 @@ static int launch_spmv(const void* indptr, const void* indices, const void* data
       FSP_SPMV_LAUNCH(8);
     }
 #undef FSP_SPMV_LAUNCH
+#undef FSP_SPMV_KERNEL
     const int st = check_launch("spmv_bins");
     if (st) return st;
   }

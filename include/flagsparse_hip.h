@@ -83,11 +83,11 @@ typedef struct fsp_csr_plan {
                                      on MI355X: neighbouring rows share an L2) */
   int64_t variant;                /* SpMV loop: 0 = software-pipelined (next trip's
                                      indices load under this trip's x gathers),
-                                     1 = plain loop, 2 = plain loop with x staged
-                                     in LDS by 1024-thread workgroups (x_len *
-                                     element size <= 64 KiB; persistent grid,
-                                     max_blocks 0 = two per CU); other kernels
-                                     ignore it */
+                                     1 = plain loop, 2 = plain and 3 = pipelined
+                                     loop with x staged in LDS by 1024-thread
+                                     workgroups (x_len * element size <= 64 KiB;
+                                     persistent grid, max_blocks 0 = two per
+                                     CU); other kernels ignore it */
 } fsp_csr_plan;
 
 /* ---- library ------------------------------------------------------------ */

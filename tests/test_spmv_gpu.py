@@ -235,7 +235,8 @@ def test_spmv_int64_auto_does_not_fallback_when_index_exceeds_int32(device, monk
 
 @pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("max_blocks,xcd", [(0, 0), (512, 0), (0, 8), (1024, 8)])
-def test_spmv_loop_variants_and_grids_bitwise_equal(device, variant, max_blocks, xcd):
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
+def test_spmv_loop_variants_and_grids_bitwise_equal(device, variant, max_blocks, xcd, dtype):
     """The pipelined (0) and plain (1) SpMV loops, one-tile-per-workgroup,
     persistent and XCD-region grids all reduce every row in the same order."""
     from flagsparse_amd import _plan
@@ -244,24 +245,26 @@ def test_spmv_loop_variants_and_grids_bitwise_equal(device, variant, max_blocks,
     ip, ix, v = long_row_matrix(5000, 3000, long_rows=(7, 4000), long_len=2900, short_len=9, seed=21)
     pl_ip, pl_ix, pl_v, _ = synthetic.power_law_rows(20000, 20000, 2.1, 24, 4096, n_dense=0, seed=5)
     for ip_, ix_, v_, shape in ((ip, ix, v, (5000, 3000)), (pl_ip, pl_ix, pl_v, (20000, 20000))):
-        data, idx, ptr = device_csr(ip_, ix_, v_, device)
+        data, idx, ptr = device_csr(ip_, ix_, v_, device, dtype)
         h = CsrHandle(idx, ptr, shape)
         x = np.random.default_rng(3).uniform(-1, 1, shape[1])
-        xt = synthetic.dense(x, device)
+        xt = synthetic.dense(x, device, dtype)
         cfg = _plan.spmv_config(unroll=4)
         ref = h.spmv(data, xt, config=cfg)
         got = h.spmv(data, xt, config=cfg, max_blocks=max_blocks, xcd_groups=xcd, variant=variant)
         assert torch.equal(got, ref)
-        assert_rel(got, oracle.spmv(ip_, ix_, v_, x, shape), torch.float32, "spmv variant")
+        gold = oracle.spmv(ip_, ix_, np_dtype_round(v_, dtype), np_dtype_round(x, dtype), shape)
+        assert_rel(got, gold, dtype, "spmv variant")
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.bfloat16, torch.float16],
                          ids=["f32", "f64", "bf16", "f16"])
 @pytest.mark.parametrize("unroll", [2, 4, 8])
 def test_spmv_x_in_lds_variant_bitwise_equal_to_plain_loop(device, dtype, unroll):
-    """Variant 2 (x staged in LDS) runs the plain loop's arithmetic per lane:
-    bitwise equal to variant 1 on every grid, for CSR and CSC (trans) launches,
-    with long-row segments, and within tolerance of the oracle."""
+    """Variants 2 and 3 (x staged in LDS; plain and pipelined loops) run the
+    same arithmetic per lane as the global-x loops: equal to variant 1 on every
+    grid, for CSR and CSC (trans) launches, with long-row segments, and within
+    tolerance of the oracle."""
     from flagsparse_amd import _plan
     from flagsparse_amd._matrix import CsrHandle
 
@@ -279,10 +282,11 @@ def test_spmv_x_in_lds_variant_bitwise_equal_to_plain_loop(device, dtype, unroll
             x = np.random.default_rng(3).uniform(-1, 1, n_x)
             xt = synthetic.dense(x, device, dtype)
             ref = h.spmv(data, xt, transpose=trans, config=cfg, variant=1)
-            for max_blocks, xcd in ((0, 0), (256, 0), (37, 8), (0, 8)):
-                got = h.spmv(data, xt, transpose=trans, config=cfg, max_blocks=max_blocks, xcd_groups=xcd,
-                             variant=2)
-                assert torch.equal(got, ref), (shape, trans, max_blocks, xcd)
+            for variant in (2, 3):
+                for max_blocks, xcd in ((0, 0), (256, 0), (37, 8), (0, 8)):
+                    got = h.spmv(data, xt, transpose=trans, config=cfg, max_blocks=max_blocks, xcd_groups=xcd,
+                                 variant=variant)
+                    assert torch.equal(got, ref), (shape, trans, variant, max_blocks, xcd)
             gold = oracle.spmv(ip_, ix_, np_dtype_round(v_, dtype), np_dtype_round(x, dtype), shape,
                                "trans" if trans else "non")
             assert_rel(ref, gold, dtype, f"ldsx {shape} trans={trans}")
@@ -294,8 +298,9 @@ def test_spmv_x_in_lds_rejects_large_x_and_default_route(device):
     ip, ix, v = synthetic.random_csr(64, 20000, 0.01, seed=4)
     data, idx, ptr = device_csr(ip, ix, v, device)
     h = CsrHandle(idx, ptr, (64, 20000))
-    with pytest.raises(RuntimeError, match="LDS"):
-        h.spmv(data, torch.ones(20000, device=device), variant=2)
+    for variant in (2, 3):
+        with pytest.raises(RuntimeError, match="LDS"):
+            h.spmv(data, torch.ones(20000, device=device), variant=variant)
     # the public entry point takes the LDS loop for a large matrix with a small x
     ip, ix, v, x = synthetic.uniform_rows(65536, 16384, 16, seed=0)
     data, idx, ptr = device_csr(ip, ix, v, device)
