@@ -16,7 +16,7 @@ from ._build import LIB_PATH
 FSP_F32, FSP_F64, FSP_F16, FSP_BF16, FSP_F8E4M3 = 0, 1, 2, 3, 4
 FSP_I32, FSP_I64 = 0, 1
 FSP_MAX_BINS = 8
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 _VALUE_CODES = {
     torch.float32: FSP_F32,
@@ -67,6 +67,7 @@ SIGNATURES = {
     "fsp_last_error": (ctypes.c_char_p, []),
     "fsp_device_arch": (_c_i32, [ctypes.c_char_p, _c_i64]),
     "fsp_csr_row_bins_workspace": (_c_i32, [_c_i64, _c_i32, ctypes.POINTER(_c_i64)]),
+    "fsp_csr_inspect": (_c_i32, [_c_i32, _c_vp, _c_i32, _c_vp, _c_i64, _c_i64, _c_vp, _c_vp]),
     "fsp_csr_row_bins": (
         _c_i32,
         [_c_i32, _c_vp, _c_i64, _c_i32, ctypes.POINTER(_c_i64), _c_vp, _c_vp, _c_vp, _c_i64, _c_vp],

@@ -206,14 +206,13 @@ class CsrPlan:
         return st
 
 
-def build_plan(indptr, n_rows, config):
-    """Bin the rows described by `indptr` (device tensor, int32/int64)."""
+def _launch_row_bins(indptr, n_rows, config, offsets):
+    """Launch the row binning of `indptr` (device tensor, int32/int64) into
+    `offsets` (device int64[n_bins + 1]); returns the row-id buffer."""
     device = indptr.device
-    n_rows = int(n_rows)
     nb = len(config.uppers) + 1  # regular bins + the long bin
     lib = _lib.load()
     rows = torch.empty(max(1, n_rows), dtype=torch.int32, device=device)
-    offsets = torch.empty(nb + 1, dtype=torch.int64, device=device)
     ws_bytes = ctypes.c_int64(0)
     _lib.check(lib.fsp_csr_row_bins_workspace(n_rows, nb, ctypes.byref(ws_bytes)), "fsp_csr_row_bins_workspace")
     ws = torch.empty(max(1, ws_bytes.value), dtype=torch.uint8, device=device)
@@ -225,7 +224,54 @@ def build_plan(indptr, n_rows, config):
         ),
         "fsp_csr_row_bins",
     )
-    off = offsets.cpu().tolist()  # host sync (prepare time only)
+    return rows
+
+
+def build_plan(indptr, n_rows, config):
+    """Bin the rows described by `indptr` (device tensor, int32/int64)."""
+    n_rows = int(n_rows)
+    offsets = torch.empty(len(config.uppers) + 2, dtype=torch.int64, device=indptr.device)
+    rows = _launch_row_bins(indptr, n_rows, config, offsets)
+    return _finish_plan(indptr, n_rows, config, rows, offsets.cpu().tolist())  # host sync (prepare time only)
+
+
+def inspect_csr(indptr, indices, n_rows, nnz, config=None):
+    """Validation figures of a CSR and, with `config`, its row-bin plan, for
+    ONE host readback.  Returns (stats, plan): stats has indptr_first,
+    indptr_last, n_decreasing, min_col, max_col (None without nonzeros) and
+    max_row; plan is None without `config` or when the row pointer is not a
+    valid CSR (the caller raises on the stats first)."""
+    device = indptr.device
+    n_rows, nnz = int(n_rows), int(nnz)
+    nb = len(config.uppers) + 1 if config is not None else 0
+    meta = torch.empty(6 + (nb + 1 if config is not None else 0), dtype=torch.int64, device=device)
+    _lib.check(
+        _lib.load().fsp_csr_inspect(
+            _index_code(indptr), _lib.ptr(indptr), _index_code(indices), _lib.ptr(indices) if nnz else None,
+            n_rows, nnz, meta.data_ptr(), _lib.stream_ptr(device),
+        ),
+        "fsp_csr_inspect",
+    )
+    rows = _launch_row_bins(indptr, n_rows, config, meta[6:]) if config is not None else None
+    host = meta.cpu().tolist()  # the one host sync
+    stats = {
+        "indptr_first": host[0],
+        "indptr_last": host[1],
+        "n_decreasing": host[2],
+        "min_col": host[3] if nnz else None,
+        "max_col": host[4] if nnz else None,
+        "max_row": host[5] if n_rows else 0,
+    }
+    plan = None
+    if config is not None and host[0] == 0 and host[1] == nnz and host[2] == 0:
+        plan = _finish_plan(indptr, n_rows, config, rows, host[6:])
+    return stats, plan
+
+
+def _finish_plan(indptr, n_rows, config, rows, off):
+    """The plan from the bin offsets `off` (host list) of a valid CSR."""
+    device = indptr.device
+    nb = len(config.uppers) + 1
     counts = [off[i + 1] - off[i] for i in range(nb)]
     regular = counts[:-1]
     n_long = counts[-1]
@@ -242,8 +288,8 @@ def build_plan(indptr, n_rows, config):
         long_rows = rows[off[nb - 1]: off[nb]].clone()
         lr = long_rows.to(torch.int64)
         ip = indptr.to(torch.int64)
-        starts = ip[lr].cpu().numpy()
-        ends = ip[lr + 1].cpu().numpy()
+        host = torch.stack([lr, ip[lr], ip[lr + 1]]).cpu().numpy()  # one readback
+        host_rows, starts, ends = host[0].astype(np.int32), host[1], host[2]
         lens = ends - starts
         seg_len = config.seg_len
         nseg = (lens + seg_len - 1) // seg_len
@@ -255,7 +301,6 @@ def build_plan(indptr, n_rows, config):
         # even split: segment j of a row covers [len*j/n, len*(j+1)/n)
         b = starts[owner] + (lens[owner] * j) // nseg[owner]
         e = starts[owner] + (lens[owner] * (j + 1)) // nseg[owner]
-        host_rows = long_rows.cpu().numpy().astype(np.int32)
         seg_row = torch.from_numpy(host_rows[owner].copy()).to(device)
         seg_begin = torch.from_numpy(b).to(device)
         seg_end = torch.from_numpy(e).to(device)

@@ -282,6 +282,73 @@ static int with_key_perm(int64_t n_cols, int32_t perm_type, F&& f) {
   return fail(FSP_ERR_DTYPE, "unsupported perm type %d", (int)perm_type);
 }
 
+// ---------------------------------------------------------------- inspect
+// Validation figures of a CSR in one pass, for one host readback:
+// stats[0] = indptr[0], stats[1] = indptr[n_rows], stats[2] = number of rows
+// with indptr[i+1] < indptr[i], stats[3] / stats[4] = smallest / largest
+// column index (INT64_MAX / INT64_MIN without nonzeros), stats[5] = longest
+// row.  min / max / counts are order-independent, so the result is
+// deterministic although waves combine with atomics.
+constexpr int64_t kStatsMin = INT64_MAX;
+constexpr int64_t kStatsMax = INT64_MIN;
+
+__global__ __launch_bounds__(64) void stats_init_kernel(int64_t* __restrict__ stats) {
+  if (threadIdx.x == 0) {
+    stats[2] = 0;
+    stats[3] = kStatsMin;
+    stats[4] = kStatsMax;
+    stats[5] = 0;
+  }
+}
+
+__device__ __forceinline__ int64_t wave_min(int64_t v) {
+  for (int m = 32; m > 0; m >>= 1) {
+    const int64_t o = __shfl_xor(v, m);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ int64_t wave_max(int64_t v) {
+  for (int m = 32; m > 0; m >>= 1) {
+    const int64_t o = __shfl_xor(v, m);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+template <typename PT, typename IT>
+__global__ __launch_bounds__(FSP_WG_THREADS) void csr_stats_kernel(const PT* __restrict__ indptr,
+                                                                   const IT* __restrict__ indices, int64_t n_rows,
+                                                                   int64_t nnz, int64_t* __restrict__ stats) {
+  const int64_t tid = static_cast<int64_t>(blockIdx.x) * FSP_WG_THREADS + threadIdx.x;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * FSP_WG_THREADS;
+  if (tid == 0) {
+    stats[0] = static_cast<int64_t>(indptr[0]);
+    stats[1] = static_cast<int64_t>(indptr[n_rows]);
+  }
+  int64_t dec = 0, longest = 0, lo = kStatsMin, hi = kStatsMax;
+  for (int64_t i = tid; i < n_rows; i += stride) {
+    const int64_t len = static_cast<int64_t>(indptr[i + 1]) - static_cast<int64_t>(indptr[i]);
+    dec += len < 0 ? 1 : 0;
+    longest = len > longest ? len : longest;
+  }
+  for (int64_t p = tid; p < nnz; p += stride) {
+    const int64_t c = static_cast<int64_t>(indices[p]);
+    lo = c < lo ? c : lo;
+    hi = c > hi ? c : hi;
+  }
+  for (int m = 32; m > 0; m >>= 1) dec += __shfl_xor(dec, m);
+  longest = wave_max(longest);
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  if ((threadIdx.x & 63) == 0) {
+    if (dec) __hip_atomic_fetch_add(&stats[2], dec, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lo != kStatsMin) __hip_atomic_fetch_min(&stats[3], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (hi != kStatsMax) __hip_atomic_fetch_max(&stats[4], hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (longest) __hip_atomic_fetch_max(&stats[5], longest, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 template <typename QT>
 static int launch_gather(int32_t elem_bytes, const QT* perm, const void* in, void* out, int64_t n, hipStream_t s) {
   const dim3 grid(grid_for(n)), block(FSP_WG_THREADS);
@@ -315,6 +382,31 @@ int fsp_device_arch(char* buf, int64_t buf_len) {
   e = hipGetDeviceProperties(&prop, dev);
   if (e != hipSuccess) return fail(FSP_ERR_LAUNCH, "hipGetDeviceProperties: %s", hipGetErrorString(e));
   snprintf(buf, static_cast<size_t>(buf_len), "%s", prop.gcnArchName);
+  return FSP_OK;
+}
+
+int fsp_csr_inspect(int32_t indptr_type, const void* indptr_dev, int32_t index_type, const void* indices_dev,
+                    int64_t n_rows, int64_t nnz, int64_t* stats_dev, void* stream) {
+  if (n_rows < 0 || nnz < 0 || indptr_dev == nullptr || stats_dev == nullptr ||
+      (nnz > 0 && indices_dev == nullptr)) {
+    return fail(FSP_ERR_ARG, "fsp_csr_inspect: bad arguments");
+  }
+  const hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(stats_init_kernel, dim3(1), dim3(64), 0, s, stats_dev);
+  int st = check_launch("csr_stats_init");
+  if (st) return st;
+  const int64_t work = n_rows > nnz ? n_rows : nnz;
+  int64_t g = ceil_div(work, FSP_WG_THREADS * 8);
+  if (g < 1) g = 1;
+  if (g > 2048) g = 2048;
+  FSP_DISPATCH_INDEX(indptr_type, PT, {
+    FSP_DISPATCH_INDEX(index_type, IT, {
+      hipLaunchKernelGGL((csr_stats_kernel<PT, IT>), dim3(static_cast<unsigned>(g)), dim3(FSP_WG_THREADS), 0, s,
+                         static_cast<const PT*>(indptr_dev), static_cast<const IT*>(indices_dev), n_rows, nnz,
+                         stats_dev);
+      return check_launch("csr_stats");
+    });
+  });
   return FSP_OK;
 }
 

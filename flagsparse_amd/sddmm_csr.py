@@ -12,6 +12,7 @@ import time
 
 import torch
 
+from . import _plan
 from ._matrix import CsrHandle
 
 SUPPORTED_SDDMM_VALUE_DTYPES = (torch.float32, torch.float64)
@@ -79,17 +80,16 @@ def _prepare_sddmm_csr_pattern(indices, indptr, shape):
     indptr64 = indptr.to(torch.int64).contiguous()
     indices = indices.contiguous()
     nnz = int(indices.numel())
-    if indptr64.numel() > 0 and int(indptr64[0].item()) != 0:
+    # every check from one device inspection and one host readback
+    stats, _ = _plan.inspect_csr(indptr64, indices, n_rows, nnz)
+    if stats["indptr_first"] != 0:
         raise ValueError("indptr[0] must be 0")
-    if indptr64.numel() > 0 and int(indptr64[-1].item()) != nnz:
+    if stats["indptr_last"] != nnz:
         raise ValueError(f"indptr[-1] must equal nnz={nnz}")
-    if indptr64.numel() > 1 and bool(torch.any(indptr64[1:] < indptr64[:-1]).item()):
+    if stats["n_decreasing"]:
         raise ValueError("indptr must be nondecreasing")
-    if nnz > 0:
-        min_col = int(indices.min().item())
-        max_col = int(indices.max().item())
-        if min_col < 0 or max_col >= n_cols:
-            raise IndexError("indices out of range for shape[1]")
+    if nnz > 0 and (stats["min_col"] < 0 or stats["max_col"] >= n_cols):
+        raise IndexError("indices out of range for shape[1]")
     return indices, indptr64, (n_rows, n_cols)
 
 

@@ -24,6 +24,7 @@ from ._common import (
     OP_NAMES,
     OP_NON,
     OP_TRANS,
+    KERNEL_VALUE_DTYPES,
     SUPPORTED_INDEX_DTYPES,
     SUPPORTED_VALUE_DTYPES,
     _INDEX_LIMIT_INT32,
@@ -61,6 +62,13 @@ def _prepare_spmm_csr_matrix(data, indices, indptr, shape):
     """spmm_csr.py:342-404 (same checks and messages).  Unlike the reference,
     index dtypes are kept native: the HIP kernels read int32 and int64
     indices directly, so no narrowed copy is made."""
+    return _inspect_spmm_csr_matrix(data, indices, indptr, shape)[0]
+
+
+def _inspect_spmm_csr_matrix(data, indices, indptr, shape, plan_config=None):
+    """_prepare_spmm_csr_matrix plus, with `plan_config`, the row-bin plan,
+    from one device inspection and ONE host readback.  Returns (validated
+    tuple, plan or None)."""
     if len(shape) != 2:
         raise ValueError("shape must be a 2-tuple: (n_rows, n_cols)")
     if data.ndim != 1 or indices.ndim != 1 or indptr.ndim != 1:
@@ -83,25 +91,23 @@ def _prepare_spmm_csr_matrix(data, indices, indptr, shape):
     if indptr.dtype not in SUPPORTED_INDEX_DTYPES:
         raise TypeError("indptr dtype must be torch.int32 or torch.int64")
     nnz = data.numel()
-    if indptr.numel() > 0 and int(indptr[0].item()) != 0:
-        raise ValueError("indptr[0] must be 0")
-    if indptr.numel() > 0 and int(indptr[-1].item()) != nnz:
-        raise ValueError(f"indptr[-1] must equal nnz={nnz}, got {int(indptr[-1].item())}")
-    if indptr.numel() > 1 and bool(torch.any(indptr[1:] < indptr[:-1]).item()):
-        raise ValueError("indptr must be nondecreasing")
-    if nnz > 0:
-        min_col = int(indices.min().item())
-        max_col = int(indices.max().item())
-        if min_col < 0 or max_col >= n_cols:
-            raise IndexError("indices out of range for n_cols")
-        if max_col > _INDEX_LIMIT_INT32:
-            raise ValueError("column indices exceed the int32 range supported by the SpMM kernel")
     data = data.contiguous()
     indices = indices.contiguous()
     indptr = indptr.contiguous()
+    stats, plan = _plan.inspect_csr(indptr, indices, n_rows, nnz, plan_config)
+    if stats["indptr_first"] != 0:
+        raise ValueError("indptr[0] must be 0")
+    if stats["indptr_last"] != nnz:
+        raise ValueError(f"indptr[-1] must equal nnz={nnz}, got {stats['indptr_last']}")
+    if stats["n_decreasing"]:
+        raise ValueError("indptr must be nondecreasing")
+    if nnz > 0:
+        if stats["min_col"] < 0 or stats["max_col"] >= n_cols:
+            raise IndexError("indices out of range for n_cols")
+        if stats["max_col"] > _INDEX_LIMIT_INT32:
+            raise ValueError("column indices exceed the int32 range supported by the SpMM kernel")
     row_lengths = indptr[1:] - indptr[:-1] if n_rows > 0 else indptr.new_empty((0,))
-    max_row_nnz = int(row_lengths.max().item()) if n_rows > 0 else 0
-    return data, indices, indptr, n_rows, n_cols, row_lengths, max_row_nnz
+    return (data, indices, indptr, n_rows, n_cols, row_lengths, stats["max_row"]), plan
 
 
 def _transpose_csr_for_spmm(data, indices, indptr, shape):
@@ -156,12 +162,23 @@ def _hint_config(lanes, block_nnz):
     return _plan.spmm_config(lanes, unroll=unroll)
 
 
+def _base_lanes(n_dense, elem_bytes, block_n):
+    return _hint_lanes(block_n, n_dense, elem_bytes) or _plan.lanes_for_dense(n_dense, elem_bytes)
+
+
+def _base_plan_config(n_dense, elem_bytes, block_n, block_nnz):
+    """The plan configuration _hip_spmm_csr_impl launches with."""
+    lanes = _base_lanes(n_dense, elem_bytes, block_n)
+    return _hint_config(lanes, block_nnz) or _plan.spmm_config(lanes)
+
+
 def _hip_spmm_csr_impl(handle, data, B, transpose=False, out=None, block_n=None, block_nnz=None):
     """Run fsp_spmm_csr (replaces _triton_spmm_csr_impl, spmm_csr.py:3373-3466)."""
     require_kernel_dtype(data.dtype)
     n_dense = int(B.shape[1])
-    lanes = _hint_lanes(block_n, n_dense, data.element_size()) or _plan.lanes_for_dense(n_dense, data.element_size())
-    return handle.spmm(data, B, transpose=transpose, out=out, lanes_n=lanes, config=_hint_config(lanes, block_nnz))
+    lanes = _base_lanes(n_dense, data.element_size(), block_n)
+    return handle.spmm(data, B, transpose=transpose, out=out, lanes_n=lanes,
+                       config=_base_plan_config(n_dense, data.element_size(), block_n, block_nnz))
 
 
 # ------------------------------------------------------------------ public API
@@ -195,12 +212,17 @@ def flagsparse_spmm_csr(
         raise ValueError("block_nnz must be positive when provided")
     if max_segments is not None and max_segments <= 0:
         raise ValueError("max_segments must be positive when provided")
-    (data, indices, indptr, n_rows, n_cols, row_lengths, max_row_nnz) = _prepare_spmm_csr_matrix(
-        data, indices, indptr, shape
+    transposes = _spmm_op_transposes(op_code)
+    # the row-bin plan of the kernel is built with the validation readback
+    # (op non only: trans runs over the CSC and plans that)
+    plan_cfg = None
+    if not transposes and torch.is_tensor(B) and B.ndim == 2 and data.dtype in KERNEL_VALUE_DTYPES:
+        plan_cfg = _base_plan_config(int(B.shape[1]), data.element_size(), block_n, block_nnz)
+    (data, indices, indptr, n_rows, n_cols, row_lengths, max_row_nnz), plan = _inspect_spmm_csr_matrix(
+        data, indices, indptr, shape, plan_cfg
     )
     if B.ndim != 2:
         raise ValueError("B must be a 2D dense tensor")
-    transposes = _spmm_op_transposes(op_code)
     eff_rows, eff_cols = (n_cols, n_rows) if transposes else (n_rows, n_cols)
     if B.shape[0] != eff_cols:
         raise ValueError(f"B.shape[0] must be n_cols={eff_cols}, got {B.shape[0]}")
@@ -223,6 +245,8 @@ def flagsparse_spmm_csr(
     timer = _Timer(return_time or return_meta)
     timer.mark()
     handle = CsrHandle(indices, indptr, (n_rows, n_cols))
+    if plan is not None:
+        handle._plans[plan.config] = plan
     values = data
     if transposes:
         handle.csc()  # the transposed structure: one radix sort, values stay in place

@@ -153,6 +153,13 @@ def _transpose_csr_for_spmv(data, indices, indptr, shape):
 
 def _prepare_spmv_csr_matrix(data, indices, indptr, shape, index_fallback_policy="auto"):
     """Validation contract of spmv_csr.py:539-595 (same order and messages)."""
+    return _inspect_spmv_csr_matrix(data, indices, indptr, shape, index_fallback_policy)[0]
+
+
+def _inspect_spmv_csr_matrix(data, indices, indptr, shape, index_fallback_policy="auto", plan_config=None):
+    """_prepare_spmv_csr_matrix plus, with `plan_config`, the row-bin plan,
+    from one device inspection and ONE host readback (the reference makes a
+    host sync per check).  Returns (validated tuple, plan or None)."""
     _normalize_spmv_index_fallback_policy(index_fallback_policy)
     if not all(torch.is_tensor(t) for t in (data, indices, indptr)):
         raise TypeError("data, indices, indptr must all be torch.Tensor")
@@ -177,24 +184,18 @@ def _prepare_spmv_csr_matrix(data, indices, indptr, shape, index_fallback_policy
     data = data.contiguous()
     indices = indices.contiguous()
     indptr = indptr.contiguous()
-
-    if indptr.numel() > 0:
-        if int(indptr[0].item()) != 0:
-            raise ValueError("indptr must start at zero")
-        if int(indptr[-1].item()) != data.numel():
-            raise ValueError("indptr[-1] must equal nnz")
-        if indptr.numel() > 1 and torch.any(indptr[1:] < indptr[:-1]).item():
-            raise ValueError("indptr must be non-decreasing")
-
     nnz = data.numel()
-    if nnz > 0:
-        min_index = int(indices.min().item())
-        max_index = int(indices.max().item())
-        if min_index < 0 or max_index >= n_cols:
-            raise IndexError("indices out of range for n_cols")
+    stats, plan = _plan.inspect_csr(indptr, indices, n_rows, nnz, plan_config)
+    if stats["indptr_first"] != 0:
+        raise ValueError("indptr must start at zero")
+    if stats["indptr_last"] != nnz:
+        raise ValueError("indptr[-1] must equal nnz")
+    if stats["n_decreasing"]:
+        raise ValueError("indptr must be non-decreasing")
+    if nnz > 0 and (stats["min_col"] < 0 or stats["max_col"] >= n_cols):
+        raise IndexError("indices out of range for n_cols")
     row_lengths = indptr[1:] - indptr[:-1]
-    max_row_nnz = int(row_lengths.max().item()) if n_rows > 0 else 0
-    return data, indices, indptr, n_rows, n_cols, row_lengths, max_row_nnz
+    return (data, indices, indptr, n_rows, n_cols, row_lengths, stats["max_row"]), plan
 
 
 def _validate_spmv_x(x, prepared):
@@ -239,8 +240,8 @@ def prepare_spmv_csr(
         if op_code == SPMV_OP_CONJ_TRANS and _is_complex_dtype(data.dtype):
             data = data.conj().resolve_conj()
         data, indices, indptr, shape = _transpose_csr_for_spmv(data, indices, indptr, shape)
-    (data, kernel_indices, kernel_indptr, n_rows, n_cols, row_lengths, max_row_nnz) = _prepare_spmv_csr_matrix(
-        data, indices, indptr, shape, index_fallback_policy=index_fallback_policy
+    (data, kernel_indices, kernel_indptr, n_rows, n_cols, row_lengths, max_row_nnz), plan = _inspect_spmv_csr_matrix(
+        data, indices, indptr, shape, index_fallback_policy=index_fallback_policy, plan_config=_plan.spmv_config()
     )
     block_nnz_use = block_nnz
     if max_segments is None:
@@ -250,7 +251,7 @@ def prepare_spmv_csr(
             max_segments_use = max((max_row_nnz + block_nnz_use - 1) // block_nnz_use, 1)
     else:
         max_segments_use = max_segments
-    return PreparedCsrSpmv(
+    prepared = PreparedCsrSpmv(
         data=data,
         kernel_indices=kernel_indices,
         kernel_indptr=kernel_indptr,
@@ -267,6 +268,8 @@ def prepare_spmv_csr(
         op=op_code,
         index_fallback_policy=index_fallback_policy,
     )
+    prepared._plan = plan  # built with the validation readback
+    return prepared
 
 
 def _hip_spmv_csr_impl_prepared(prepared, x, out=None):

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSP_ABI_VERSION 7
+#define FSP_ABI_VERSION 8
 #define FSP_MAX_BINS 8
 
 typedef enum fsp_value_type {
@@ -104,6 +104,17 @@ int fsp_device_arch(char* buf, int64_t buf_len);
  * upper[b-1] < len <= upper[b] (upper[-1] = -1, upper[n_bins-1] = +inf).
  * rows_out_dev[n_rows] receives the row ids bin after bin (ascending inside
  * each bin); bin_offsets_out_dev[n_bins+1] the bin boundaries. */
+/* Validation figures of a CSR in one launch (replaces the per-check host
+ * syncs of _prepare_spmv_csr_matrix, spmv_csr.py:539-595, and
+ * _prepare_spmm_csr_matrix, spmm_csr.py:342-404): stats_dev[6] receives
+ * indptr[0], indptr[n_rows], the number of rows with indptr[i+1] <
+ * indptr[i], the smallest and largest column index (INT64_MAX / INT64_MIN
+ * when nnz = 0) and the longest row length.  Reads only indptr[0..n_rows]
+ * and indices[0..nnz). */
+int fsp_csr_inspect(int32_t indptr_type, const void* indptr_dev, int32_t index_type,
+                    const void* indices_dev, int64_t n_rows, int64_t nnz,
+                    int64_t* stats_dev, void* stream);
+
 int fsp_csr_row_bins_workspace(int64_t n_rows, int32_t n_bins, int64_t* bytes);
 int fsp_csr_row_bins(int32_t indptr_type, const void* indptr_dev, int64_t n_rows,
                      int32_t n_bins, const int64_t* upper,
