@@ -111,7 +111,39 @@ def bench_spmm_power_law(dev, steps, warmup):
     }
 
 
+def graph_time_ms(fn, dev, per_graph=64, reps=15):
+    """Median device time per call of `fn` replayed from a captured graph of
+    `per_graph` calls: the kernels back to back, without host launch gaps."""
+    s = torch.cuda.Stream(dev)
+    s.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(s):
+        fn()
+    s.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        for _ in range(per_graph):
+            fn()
+    times = []
+    with torch.cuda.stream(s):
+        g.replay()
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(s)
+            g.replay()
+            b.record(s)
+            b.synchronize()
+            times.append(a.elapsed_time(b) / per_graph)
+    s.synchronize()
+    return statistics.median(times)
+
+
 def bench_spmv_uniform(dev, steps, warmup):
+    """16384^2 x 16 SpMV (2.3 MB, cache-resident after the first call): host
+    wall clock per call, and device time per call from graph replay (the
+    launch-latency-free figure; bytes / that = cache-resident GB/s)."""
+    from flagsparse_amd import autotune  # noqa: PLC0415
+
     ip, ix, v, x = synthetic.uniform_rows(16384, 16384, 16, seed=0)
     data, idx, ptr = synthetic.to_torch(ip, ix, v, dev, torch.float32, torch.int32)
     xt = synthetic.dense(x, dev, torch.float32)
@@ -121,9 +153,40 @@ def bench_spmv_uniform(dev, steps, warmup):
     stream = torch.cuda.current_stream(dev)
     wall_ms, ev = time_steps(lambda: handle.spmv(data, xt, out=y), steps * 5, warmup, stream)
     nbytes = ix.size * 8 + 16385 * 4 + 16384 * 4 * 2
+    graph_ms = graph_time_ms(lambda: handle.spmv(data, xt, out=y), dev)
+    best = None
+    for p in autotune.candidates("spmv", 1, 4, 16384):  # graph-timed: launch gaps would hide the kernels
+        ms = graph_time_ms(lambda: handle.spmv(data, xt, out=y, config=p.spmv_config(), max_blocks=p.max_blocks(dev),
+                                               xcd_groups=p.xcd_groups, variant=p.variant), dev, 32, 5)
+        if best is None or ms < best[0]:
+            best = (ms, p)
+    return {"ms": wall_ms / (steps * 5), "event_ms_median": statistics.median(ev), "bytes": nbytes,
+            "graph_ms_per_call": graph_ms, "gbs_cache_resident": nbytes / (graph_ms * 1e-3) / 1e9,
+            "best_graph_ms_per_call": best[0], "best_params": best[1].__dict__,
+            "best_gbs_cache_resident": nbytes / (best[0] * 1e-3) / 1e9}
+
+
+def bench_spmm_tuned(dev, steps, warmup, inputs):
+    """The headline SpMM with the autotuner's parameters for its shape class."""
+    from flagsparse_amd import autotune  # noqa: PLC0415
+
+    ip, ix, v, B = inputs
+    n = 262144
+    data, idx, ptr = synthetic.to_torch(ip, ix, v, dev, torch.float32, torch.int32)
+    Bt = synthetic.dense(B, dev, torch.float32)
+    handle = CsrHandle(idx, ptr, (n, n))
+    p = autotune.tune_spmm(handle, data, 64)
+    ln = p.lanes_n or _plan.lanes_for_dense(64, 4)
+    cfg = p.spmm_config(ln)
+    C = torch.empty((n, 64), dtype=torch.float32, device=dev)
+
+    def step():
+        handle.spmm(data, Bt, out=C, lanes_n=ln, config=cfg, max_blocks=p.max_blocks(dev), xcd_groups=p.xcd_groups)
+
+    _, ev = time_steps(step, steps, warmup, torch.cuda.current_stream(dev))
     med = statistics.median(ev)
-    return {"ms": wall_ms / (steps * 5), "event_ms_median": med, "bytes": nbytes,
-            "gbs_cache_resident": nbytes / (med * 1e-3) / 1e9}
+    return {"params": p.__dict__, "event_ms_median": round(med, 5),
+            "gbs": round(spmm_bytes(int(ix.size), n, n, 64) / (med * 1e-3) / 1e9, 2)}
 
 
 def bench_autograd_bf16_vs_fp32(dev, steps, warmup):
@@ -425,7 +488,8 @@ def main():
     ap.add_argument("--with-copy", action="store_true",
                     help="also run the 1 GiB copy (the counter calibration kernel) under --no-extras")
     ap.add_argument("--only", default="",
-                    help="comma-separated extras to run (copy, spmv_large, spmv_small, autograd, bsr, batch, vendor); "
+                    help="comma-separated extras to run (copy, spmv_large, spmv_small, spmm_tuned, autograd, bsr, batch, "
+                         "vendor); "
                          "default: all")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -483,7 +547,7 @@ def main():
         },
     }
     extras = set(e for e in args.only.split(",") if e) if args.only else {
-        "copy", "spmv_large", "spmv_small", "autograd", "bsr", "batch", "vendor"}
+        "copy", "spmv_large", "spmv_small", "spmm_tuned", "autograd", "bsr", "batch", "vendor"}
     if args.no_extras:
         extras = {"copy"} if args.with_copy else set()
     if "copy" in extras:
@@ -492,9 +556,9 @@ def main():
         line["spmv_uniform_16m_rows_x16"] = bench_spmv_large(dev, args.steps, args.warmup)
     if "spmv_small" in extras:
         s = bench_spmv_uniform(dev, args.steps, args.warmup)
-        line["spmv_uniform_16384x16"] = {"ms_per_step": round(s["ms"], 5),
-                                         "event_ms_median": round(s["event_ms_median"], 5),
-                                         "gbs_cache_resident": round(s["gbs_cache_resident"], 2)}
+        line["spmv_uniform_16384x16"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in s.items()}
+    if "spmm_tuned" in extras:
+        line["spmm_power_law_tuned"] = bench_spmm_tuned(dev, args.steps, args.warmup, r["inputs"])
     if "autograd" in extras:
         a = bench_autograd_bf16_vs_fp32(dev, max(5, args.steps // 5), args.warmup)
         line["bf16_fwd_bwd_speedup_vs_fp32"] = round(a["speedup_bf16_vs_fp32"], 3)
