@@ -64,10 +64,19 @@ LDSX_MAX_BYTES = 65536
 LDSX_MIN_NNZ = 1 << 20
 
 
-def spmv_default_variant(x_len, elem_bytes, nnz):
-    """SpMV loop used without tuning: 2 (x in LDS) when x fits and the matrix
-    is large, else 0 (software-pipelined)."""
-    return 2 if (0 < x_len * elem_bytes <= LDSX_MAX_BYTES and nnz >= LDSX_MIN_NNZ) else 0
+def chunk_aligned(*tensors):
+    """True when every tensor's storage starts on a 4-element boundary (the
+    vector-chunk SpMV loop reads 4-element chunks)."""
+    return all(t.data_ptr() % (4 * t.element_size()) == 0 for t in tensors)
+
+
+def spmv_default_variant(x_len, elem_bytes, nnz, chunk_ok=False):
+    """SpMV loop used without tuning: when x fits in LDS and the matrix is
+    large, 4 (vector chunks, needs 4-element aligned index / value arrays and
+    unroll 4 or 8) or else 2 (x in LDS); otherwise 0 (software-pipelined)."""
+    if 0 < x_len * elem_bytes <= LDSX_MAX_BYTES and nnz >= LDSX_MIN_NNZ:
+        return 4 if chunk_ok else 2
+    return 0
 
 
 def spmv_config(unroll=SPMV_UNROLL, long_threshold=SPMV_LONG_THRESHOLD, seg_len=SPMV_SEG_LEN):
@@ -176,7 +185,8 @@ class CsrPlan:
         xcd_groups > 1 cuts each bin's tiles into that many contiguous regions
         served by the workgroups b with b % xcd_groups == region (8 = one
         region per XCD).  variant selects the SpMV loop (0 pipelined, 1 plain,
-        2 plain and 3 pipelined with x staged in LDS)."""
+        2 plain and 3 pipelined with x staged in LDS, 4 vector chunks with x
+        in LDS)."""
         st = _lib.FspCsrPlan()
         st.n_bins = len(self.bin_counts)
         st.unroll = self.config.unroll

@@ -17,7 +17,7 @@ The tuned parameters are the ones the kernels take from their plan:
                      XCD, so neighbouring rows share an L2 (0 = off);
 * ``variant``        SpMV loop: 0 = software-pipelined, 1 = plain, 2 = plain
                      and 3 = pipelined with x staged in LDS (x of at most
-                     64 KiB).
+                     64 KiB), 4 = vector chunks with x in LDS.
 
 A shape class is (kernel, value / index dtypes, dense width, rounded log2 of
 the mean and maximum row length and of the row count, compute units).  The
@@ -98,10 +98,12 @@ def shape_class(kind, value_dtype, index_dtype, indptr_dtype, n_dense, n_rows, n
     return "|".join(parts)
 
 
-def candidates(kind, n_dense=1, elem_bytes=4, x_len=0):
+def candidates(kind, n_dense=1, elem_bytes=4, x_len=0, chunk_ok=False):
     """Candidate parameter sets of a kernel family (small, fixed grids).
     For SpMV, x_len > 0 with x_len * elem_bytes <= 64 KiB adds the loops with
-    x staged in LDS (variants 2 and 3: 1024-thread workgroups, 1 or 2 per CU)."""
+    x staged in LDS (variants 2 and 3: 1024-thread workgroups, 1 or 2 per CU)
+    and, with chunk_ok (4-element aligned arrays), the vector-chunk loop
+    (variant 4)."""
     out = []
     if kind == "spmv":
         for variant in (0, 1):
@@ -115,6 +117,11 @@ def candidates(kind, n_dense=1, elem_bytes=4, x_len=0):
                     for bpc in (1, 2):
                         for xg in (0, 8):
                             out.append(TuneParams(unroll=unroll, blocks_per_cu=bpc, xcd_groups=xg, variant=variant))
+            if chunk_ok:
+                for unroll in (4, 8):
+                    for bpc in (1, 2):
+                        for xg in (0, 8):
+                            out.append(TuneParams(unroll=unroll, blocks_per_cu=bpc, xcd_groups=xg, variant=4))
         return out
     g = _plan.lanes_for_dense(n_dense, elem_bytes)
     lanes = [g] + ([g // 2] if g >= 8 else [])
@@ -211,14 +218,15 @@ def tune_spmv(handle, data, allow_tuning=True):
     lengths = handle.indptr[1:] - handle.indptr[:-1] if handle.n_rows else handle.indptr[:0]
     max_row = int(lengths.max().item()) if handle.n_rows else 0
     x_lds = 0 < handle.n_cols * data.element_size() <= _plan.LDSX_MAX_BYTES
+    chunk = x_lds and _plan.chunk_aligned(handle.indices, data)
     key = shape_class("spmv", data.dtype, handle.indices.dtype, handle.indptr.dtype, 1, handle.n_rows, handle.nnz,
-                      max_row, compute_units(device), extra="xlds" if x_lds else "")
+                      max_row, compute_units(device), extra=("xlds-chunk" if chunk else "xlds") if x_lds else "")
     hit = lookup(key)
     if hit is not None or not allow_tuning:
         return hit or TuneParams()
     x = torch.ones(handle.n_cols, dtype=data.dtype, device=device)
     y = torch.empty(handle.n_rows, dtype=data.dtype, device=device)
-    cands = candidates("spmv", 1, data.element_size(), handle.n_cols)
+    cands = candidates("spmv", 1, data.element_size(), handle.n_cols, _plan.chunk_aligned(handle.indices, data))
     for p in cands:  # plans (host syncs) before any timing
         handle.plan(p.spmv_config())
 

@@ -190,7 +190,7 @@ static inline int check_plan(const fsp_csr_plan* plan, int64_t n_rows) {
                 (long long)total, (long long)plan->n_long_rows, (long long)n_rows);
   }
   if (plan->n_segments < 0 || plan->n_long_rows < 0) return fail(FSP_ERR_ARG, "plan: negative segment counts");
-  if (plan->variant < 0 || plan->variant > 3) return fail(FSP_ERR_ARG, "plan: unknown variant %lld", (long long)plan->variant);
+  if (plan->variant < 0 || plan->variant > 4) return fail(FSP_ERR_ARG, "plan: unknown variant %lld", (long long)plan->variant);
   if (plan->xcd_groups < 0 || plan->xcd_groups > 64) {
     return fail(FSP_ERR_ARG, "plan: xcd_groups %lld outside [0, 64]", (long long)plan->xcd_groups);
   }

@@ -285,7 +285,9 @@ def _hip_spmv_csr_impl_prepared(prepared, x, out=None):
     acc = torch.float64 if dtype in (torch.float32, torch.float64) else torch.float32
     partials = plan.partials(1, acc, device)
     nnz = int(prepared.kernel_indices.numel())
-    st = plan.struct(partials, variant=_plan.spmv_default_variant(int(x.shape[0]), x.element_size(), nnz))
+    variant = _plan.spmv_default_variant(int(x.shape[0]), x.element_size(), nnz,
+                                         _plan.chunk_aligned(prepared.kernel_indices, prepared.data))
+    st = plan.struct(partials, variant=variant)
     _lib.check(
         _lib.load().fsp_spmv_csr(
             _lib.value_code(dtype),

@@ -87,7 +87,10 @@ typedef struct fsp_csr_plan {
                                      loop with x staged in LDS by 1024-thread
                                      workgroups (x_len * element size <= 64 KiB;
                                      persistent grid, max_blocks 0 = two per
-                                     CU); other kernels ignore it */
+                                     CU), 4 = as 2 with the index / value (or
+                                     perm) streams read as 4-element chunks
+                                     (arrays aligned to 4 elements, unroll 4
+                                     or 8); other kernels ignore it */
 } fsp_csr_plan;
 
 /* ---- library ------------------------------------------------------------ */

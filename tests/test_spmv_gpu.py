@@ -309,3 +309,74 @@ def test_spmv_x_in_lds_rejects_large_x_and_default_route(device):
     assert _plan.spmv_default_variant(16384, 4, int(ix.size)) == 2
     y = fs.flagsparse_spmv_csr(data, idx, ptr, synthetic.dense(x, device), (65536, 16384))
     assert_rel(y, oracle.spmv(ip, ix, v, x, (65536, 16384)), torch.float32, "default ldsx")
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.bfloat16, torch.float16],
+                         ids=["f32", "f64", "bf16", "f16"])
+@pytest.mark.parametrize("index_dtype", [torch.int32, torch.int64], ids=["i32", "i64"])
+@pytest.mark.parametrize("unroll", [4, 8])
+def test_spmv_vector_chunk_variant(device, dtype, index_dtype, unroll):
+    """Variant 4 (4-element chunks, x in LDS): rows starting anywhere, the
+    last chunk at the arrays' end, long-row segments, CSR and CSC (through
+    the permutation) launches, every grid; within tolerance of the oracle,
+    close to the plain loop and deterministic."""
+    from flagsparse_amd import _plan
+    from flagsparse_amd._matrix import CsrHandle
+
+    n_cols = 8192 if dtype == torch.float64 else 16384
+    ip, ix, v = long_row_matrix(5001, 3000, long_rows=(7, 4000), long_len=2900, short_len=9, seed=21)
+    r_ip, r_ix, r_v = synthetic.random_csr(4093, n_cols, 0.0017, seed=9)  # ragged rows, nnz % 4 != 0 likely
+    cases = [(ip, ix, v, (5001, 3000)), (r_ip, r_ix, r_v, (4093, n_cols))]
+    # rows above 1024 nonzeros run as segments (the 2900-long rows here)
+    cfg = _plan.spmv_config(unroll=unroll, long_threshold=1024, seg_len=1024)
+    for ip_, ix_, v_, shape in cases:
+        data, idx, ptr = device_csr(ip_, ix_, v_, device, dtype, index_dtype)
+        assert _plan.chunk_aligned(idx, data)
+        assert h_segments(idx, ptr, shape, cfg) > 0 or shape[0] == 4093
+        h = CsrHandle(idx, ptr, shape)
+        for trans in (False, True):
+            n_x = shape[0] if trans else shape[1]
+            if n_x * data.element_size() > 65536:
+                continue
+            x = np.random.default_rng(3).uniform(-1, 1, n_x)
+            xt = synthetic.dense(x, device, dtype)
+            gold = oracle.spmv(ip_, ix_, np_dtype_round(v_, dtype), np_dtype_round(x, dtype), shape,
+                               "trans" if trans else "non")
+            plain = h.spmv(data, xt, transpose=trans, config=cfg, variant=1)
+            first = None
+            for max_blocks, xcd in ((0, 0), (256, 0), (37, 8)):
+                got = h.spmv(data, xt, transpose=trans, config=cfg, max_blocks=max_blocks, xcd_groups=xcd, variant=4)
+                assert_rel(got, gold, dtype, f"vec {shape} trans={trans} grid={max_blocks},{xcd}")
+                assert_rel(got, plain.to(torch.float64).cpu().numpy(), dtype, "vec vs plain")
+                if first is None:
+                    first = got
+                else:
+                    assert torch.equal(got, first)  # same per-row order on every grid
+
+
+def h_segments(idx, ptr, shape, cfg):
+    from flagsparse_amd._matrix import CsrHandle
+
+    return CsrHandle(idx, ptr, shape).plan(cfg).n_segments
+
+
+def test_spmv_vector_chunk_variant_rejects_misaligned_and_unroll(device):
+    from flagsparse_amd import _plan
+    from flagsparse_amd._matrix import CsrHandle
+
+    ip, ix, v = synthetic.random_csr(500, 700, 0.02, seed=5)
+    data, idx, ptr = device_csr(ip, ix, v, device)
+    h = CsrHandle(idx, ptr, (500, 700))
+    x = torch.ones(700, device=device)
+    with pytest.raises(RuntimeError, match="unroll 4 or 8"):
+        h.spmv(data, x, config=_plan.spmv_config(unroll=2), variant=4)
+    # a values view one element in: not on a 4-element boundary
+    big = torch.zeros(data.numel() + 1, device=device)
+    big[1:] = data
+    shifted = big[1:]
+    assert not _plan.chunk_aligned(shifted)
+    with pytest.raises(RuntimeError, match="aligned"):
+        h.spmv(shifted, x, variant=4)
+    # the untuned entry point picks the plain LDS loop for such a view
+    assert _plan.spmv_default_variant(700, 4, 1 << 20, _plan.chunk_aligned(idx, shifted)) == 2
+    assert _plan.spmv_default_variant(700, 4, 1 << 20, _plan.chunk_aligned(idx, data)) == 4
