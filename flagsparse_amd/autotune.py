@@ -211,16 +211,23 @@ def clear_memory():
 
 
 # ------------------------------------------------------------------ tuning
+def spmv_class_tag(x_len, indices, data):
+    """Shape-class tag of the SpMV candidate set: x fits in LDS (the LDS
+    loops), and then whether the arrays sit on 4-element boundaries (the
+    vector-chunk loop)."""
+    if not 0 < x_len * data.element_size() <= _plan.LDSX_MAX_BYTES:
+        return ""
+    return "xlds-chunk" if _plan.chunk_aligned(indices, data) else "xlds"
+
+
 def tune_spmv(handle, data, allow_tuning=True):
     """Parameters of y = A x for a CsrHandle (shape class of its rows): the
     cached winner, or tuned now on this matrix (x = 1, y scratch)."""
     device = data.device
     lengths = handle.indptr[1:] - handle.indptr[:-1] if handle.n_rows else handle.indptr[:0]
     max_row = int(lengths.max().item()) if handle.n_rows else 0
-    x_lds = 0 < handle.n_cols * data.element_size() <= _plan.LDSX_MAX_BYTES
-    chunk = x_lds and _plan.chunk_aligned(handle.indices, data)
     key = shape_class("spmv", data.dtype, handle.indices.dtype, handle.indptr.dtype, 1, handle.n_rows, handle.nnz,
-                      max_row, compute_units(device), extra=("xlds-chunk" if chunk else "xlds") if x_lds else "")
+                      max_row, compute_units(device), extra=spmv_class_tag(handle.n_cols, handle.indices, data))
     hit = lookup(key)
     if hit is not None or not allow_tuning:
         return hit or TuneParams()

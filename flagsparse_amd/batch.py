@@ -139,8 +139,9 @@ class CsrBatch:
         defaults."""
         dtype = dtype or self.data.dtype
         cus = autotune.compute_units(self.device)
+        tag = autotune.spmv_class_tag(self.total_cols, self.indices, self.data) if kind == "spmv" else ""
         key = autotune.shape_class(kind, dtype, self.indices.dtype, self.indptr.dtype, n_dense, self.total_rows,
-                                   self.nnz, self.max_row, cus)
+                                   self.nnz, self.max_row, cus, extra=tag)
         p = self._params.get(key) or autotune.lookup(key)
         if p is None:
             if not tune:
@@ -151,8 +152,8 @@ class CsrBatch:
 
     def _tune(self, kind, key, n_dense, dtype):
         elem = torch.empty(0, dtype=dtype).element_size()
-        cands = autotune.candidates(kind, n_dense, elem)
         data = self.data if self.data.dtype == dtype else self.data.to(dtype)
+        cands = autotune.candidates(kind, n_dense, elem, self.total_cols, _plan.chunk_aligned(self.indices, data))
         if kind == "spmv":
             x = torch.ones(self.total_cols, dtype=dtype, device=self.device)
             y = torch.empty(self.total_rows, dtype=dtype, device=self.device)
@@ -233,7 +234,11 @@ class CsrBatch:
                 n0, n1 = int(self.nnz_offsets[m]), int(self.nnz_offsets[m + 1])
                 c0, c1 = int(self.col_offsets[m]), int(self.col_offsets[m + 1])
                 h = CsrHandle(self.local_indices[n0:n1], self.local_indptr[r0 + m: r1 + m + 1], (r1 - r0, c1 - c0))
-                mats.append((h, r0, r1, n0, n1, c0, c1))
+                # the matrix's rows of the packed block-diagonal CSR: a row
+                # range of the packed row pointer over the packed column ids
+                # (global columns, the packed arrays' own alignment)
+                hg = CsrHandle(self.indices, self.indptr[r0: r1 + 1], (r1 - r0, self.total_cols))
+                mats.append((h, hg, r0, r1, n0, n1, c0, c1))
             self._per_matrix = mats
         return self._per_matrix
 
@@ -243,8 +248,10 @@ class CsrBatch:
         p = params or self.params("spmv", 1, data.dtype)
         y = out if out is not None else torch.empty(self.total_rows, dtype=data.dtype, device=self.device)
         cfg = p.spmv_config()
-        for h, r0, r1, n0, n1, c0, c1 in self._matrices():
-            h.spmv(data[n0:n1], x[c0:c1], out=y[r0:r1], config=cfg, variant=p.variant)
+        for _, hg, r0, r1, _n0, _n1, _c0, _c1 in self._matrices():
+            # one launch per matrix over the packed arrays: the same chunk
+            # boundaries and summation order as the batched launch
+            hg.spmv(data, x, out=y[r0:r1], config=cfg, variant=p.variant)
         return y
 
     def spmm_per_matrix(self, B, out=None, data=None, params=None):
@@ -255,7 +262,7 @@ class CsrBatch:
         cfg = p.spmm_config(lanes)
         C = out if out is not None else torch.empty((self.total_rows, n_dense), dtype=data.dtype,
                                                     device=self.device)
-        for h, r0, r1, n0, n1, c0, c1 in self._matrices():
+        for h, _hg, r0, r1, n0, n1, c0, c1 in self._matrices():
             h.spmm(data[n0:n1], B[c0:c1], out=C[r0:r1], lanes_n=lanes, config=cfg)
         return C
 

@@ -175,8 +175,11 @@ def test_autotune_picks_and_caches(device, tmp_path, monkeypatch):
     ip, ix, v, ro, no = S.csr_batch(n_mats=128, rows_lo=64, rows_hi=4096, seed=13, device=device)
     b = CsrBatch(v, ix, ip, ro, None, no)
     p = b.params("spmv")
-    assert p in autotune.candidates("spmv")
-    assert len(b.tune_results) == len(autotune.candidates("spmv"))
+    from flagsparse_amd import _plan
+
+    cands = autotune.candidates("spmv", 1, 4, b.total_cols, _plan.chunk_aligned(b.indices, b.data))
+    assert p in cands
+    assert len(b.tune_results) == len(cands)
     assert path.exists() and "spmv|float32" in path.read_text()
     autotune.clear_memory()
     b2 = CsrBatch(v, ix, ip, ro, None, no)
