@@ -231,12 +231,13 @@ def bench_autograd_bf16_vs_fp32(dev, steps, warmup):
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "spmm_bins_kernel_power_law_262144_n64_pmc.md")
+SPMV_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "spmv_vec_kernel_uniform_16m_x16_pmc.md")
 
 
 def measured_traffic(path=TRAFFIC_SUMMARY):
-    """HBM bytes per launch of the headline SpMM from the committed rocprofv3
-    --pmc summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy
-    kernel's calibration factor); None when the summary is absent."""
+    """HBM bytes per launch of a kernel (default: the headline SpMM) from a
+    committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE per dispatch,
+    scaled by the copy kernel's calibration factor); None when absent."""
     try:
         with open(path) as f:
             for line in f:
@@ -286,6 +287,7 @@ def bench_spmv_large(dev, steps, warmup):
     del ptr, idx, data, x, y, handle
     return {"rows": m, "cols": n, "nnz": nnz, "bytes": nbytes, "event_ms_median": round(med, 5),
             "tuned": {k: v for k, v in p.__dict__.items()},
+            "traffic": measured_traffic(SPMV_TRAFFIC_SUMMARY) if p.variant == 4 else None,
             "gbs": round(nbytes / (med * 1e-3) / 1e9, 2), "hbm_frac": round(nbytes / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
