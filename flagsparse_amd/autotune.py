@@ -17,7 +17,10 @@ The tuned parameters are the ones the kernels take from their plan:
                      XCD, so neighbouring rows share an L2 (0 = off);
 * ``variant``        SpMV loop: 0 = software-pipelined, 1 = plain, 2 = plain
                      and 3 = pipelined with x staged in LDS (x of at most
-                     64 KiB), 4 = vector chunks with x in LDS.
+                     64 KiB), 4 = vector chunks with x in LDS; for a batch,
+                     5 = row-range parts with each matrix's column range of
+                     x in LDS (``lanes_n`` lanes per row, ``part_nnz``
+                     nonzeros per part).
 
 A shape class is (kernel, value / index dtypes, dense width, rounded log2 of
 the mean and maximum row length and of the row count, compute units).  The
@@ -54,6 +57,7 @@ class TuneParams:
     blocks_per_cu: int = 0
     xcd_groups: int = 0
     variant: int = 0
+    part_nnz: int = 0  # batch SpMV variant 5: nonzeros per row-range part
 
     def spmv_config(self):
         lt = self.long_threshold or _plan.SPMV_LONG_THRESHOLD
@@ -96,6 +100,18 @@ def shape_class(kind, value_dtype, index_dtype, indptr_dtype, n_dense, n_rows, n
     if extra:
         parts.append(extra)
     return "|".join(parts)
+
+
+def batch_spmv_candidates(elem_bytes=4, x_len=0, chunk_ok=False):
+    """SpMV candidates of a batch: the single-matrix set plus, with chunk_ok,
+    the row-range parts kernel (variant 5: each matrix's column range of x
+    in LDS; lanes_n lanes per row, part_nnz nonzeros per part)."""
+    out = candidates("spmv", 1, elem_bytes, x_len, chunk_ok)
+    if chunk_ok:
+        for lanes in (4, 8, 16):
+            for part_nnz in (1 << 15, 1 << 17):
+                out.append(TuneParams(unroll=4, lanes_n=lanes, variant=5, part_nnz=part_nnz))
+    return out
 
 
 def candidates(kind, n_dense=1, elem_bytes=4, x_len=0, chunk_ok=False):

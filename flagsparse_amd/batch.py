@@ -85,6 +85,7 @@ class CsrBatch:
         self.total_rows, self.total_cols, self.nnz = int(rows[-1]), int(cols[-1]), int(nnz[-1])
         self._pack()
         self._params = {}
+        self._parts = {}
         self._per_matrix = None
 
     @classmethod
@@ -153,16 +154,22 @@ class CsrBatch:
     def _tune(self, kind, key, n_dense, dtype):
         elem = torch.empty(0, dtype=dtype).element_size()
         data = self.data if self.data.dtype == dtype else self.data.to(dtype)
-        cands = autotune.candidates(kind, n_dense, elem, self.total_cols, _plan.chunk_aligned(self.indices, data))
+        chunk_ok = _plan.chunk_aligned(self.indices, data)
+        if kind == "spmv":
+            cands = autotune.batch_spmv_candidates(elem, self.total_cols, chunk_ok)
+        else:
+            cands = autotune.candidates(kind, n_dense, elem, self.total_cols, chunk_ok)
         if kind == "spmv":
             x = torch.ones(self.total_cols, dtype=dtype, device=self.device)
             y = torch.empty(self.total_rows, dtype=dtype, device=self.device)
-            for p in cands:  # plans are built (host syncs) before any timing
-                self.handle.plan(p.spmv_config())
+            for p in cands:  # plans / parts are built (host syncs) before any timing
+                if p.variant == 5:
+                    self.parts(p.part_nnz)
+                else:
+                    self.handle.plan(p.spmv_config())
 
             def run(p):
-                self.handle.spmv(data, x, out=y, config=p.spmv_config(), max_blocks=p.max_blocks(self.device),
-                                 xcd_groups=p.xcd_groups, variant=p.variant)
+                self.spmv(x, out=y, data=data, params=p)
         else:
             B = torch.ones((self.total_cols, n_dense), dtype=dtype, device=self.device)
             C = torch.empty((self.total_rows, n_dense), dtype=dtype, device=self.device)
@@ -194,8 +201,51 @@ class CsrBatch:
         if x.ndim != 1 or x.dtype != data.dtype:
             raise ValueError("x must be 1D with the values' dtype")
         p = params or self.params("spmv", 1, data.dtype)
+        if p.variant == 5:
+            y = out if out is not None else torch.empty(self.total_rows, dtype=data.dtype, device=self.device)
+            parts, _ = self.parts(p.part_nnz)
+            self._spmv_parts(data, x.contiguous(), y, parts, p.lanes_n)
+            return y
         return self.handle.spmv(data, x.contiguous(), out=out, config=p.spmv_config(),
                                 max_blocks=p.max_blocks(self.device), xcd_groups=p.xcd_groups, variant=p.variant)
+
+    # ----------------------------------------------- row-range parts
+    def parts(self, part_nnz):
+        """(device int64 [P, 4] parts, host start index of each matrix's parts)
+        for fsp_spmv_csr_parts: every matrix's rows cut evenly into
+        ceil(nnz_m / part_nnz) row ranges, each with the matrix's column range
+        as its window (built once per part size)."""
+        key = int(part_nnz)
+        hit = self._parts.get(key)
+        if hit is not None:
+            return hit
+        rows = np.asarray(self.row_offsets, dtype=np.int64)
+        cols = np.asarray(self.col_offsets, dtype=np.int64)
+        nnz = np.asarray(self.nnz_offsets, dtype=np.int64)
+        n_rows = rows[1:] - rows[:-1]
+        n_parts = np.where(n_rows > 0, np.maximum(1, -(-(nnz[1:] - nnz[:-1]) // max(1, key))), 0)
+        n_parts = np.minimum(n_parts, n_rows)
+        start = np.concatenate([[0], np.cumsum(n_parts)])
+        owner = np.repeat(np.arange(self.n_mats, dtype=np.int64), n_parts)
+        k = np.arange(int(start[-1]), dtype=np.int64) - start[owner]
+        rb = rows[owner] + n_rows[owner] * k // n_parts[owner]
+        re = rows[owner] + n_rows[owner] * (k + 1) // n_parts[owner]
+        table = np.stack([rb, re, cols[owner], cols[owner + 1]], 1).astype(np.int64)
+        dev = torch.from_numpy(np.ascontiguousarray(table)).to(self.device)
+        self._parts[key] = (dev, start)
+        return self._parts[key]
+
+    def _spmv_parts(self, data, x, y, parts, lanes):
+        if parts.shape[0] == 0:
+            return
+        _lib.check(
+            _lib.load().fsp_spmv_csr_parts(
+                _lib.value_code(data.dtype), _lib.index_code(self.indices.dtype), _lib.index_code(self.indptr.dtype),
+                self.indptr.data_ptr(), _lib.ptr(self.indices), _lib.ptr(data), _lib.ptr(x), y.data_ptr(),
+                int(data.numel()), int(parts.shape[0]), parts.data_ptr(), int(lanes), _lib.stream_ptr(self.device),
+            ),
+            "fsp_spmv_csr_parts",
+        )
 
     def spmm(self, B, out=None, data=None, params=None):
         """C (packed by rows) = blockdiag(A_m) B (packed by rows): one launch."""
@@ -247,6 +297,11 @@ class CsrBatch:
         data = self.data if data is None else data
         p = params or self.params("spmv", 1, data.dtype)
         y = out if out is not None else torch.empty(self.total_rows, dtype=data.dtype, device=self.device)
+        if p.variant == 5:  # each matrix's own parts, one launch per matrix
+            parts, start = self.parts(p.part_nnz)
+            for m in range(self.n_mats):
+                self._spmv_parts(data, x, y, parts[int(start[m]):int(start[m + 1])], p.lanes_n)
+            return y
         cfg = p.spmv_config()
         for _, hg, r0, r1, _n0, _n1, _c0, _c1 in self._matrices():
             # one launch per matrix over the packed arrays: the same chunk

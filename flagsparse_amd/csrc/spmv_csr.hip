@@ -350,6 +350,90 @@ __global__ __launch_bounds__(kLdsxThreads) void spmv_vec_kernel(
 #undef FSP_VEC_ROWS
 }
 
+// Row-range parts (fsp_spmv_csr_parts): workgroup p runs rows
+// [row_begin, row_end) of part p, whose nonzeros use columns [col_lo, col_hi)
+// only.  It copies x[col_lo, col_hi) into LDS once -- one barrier per part --
+// when the window fits kLdsxBytes, else gathers from global memory, and runs
+// the rows with the vector-chunk loop (L lanes per row).  A batch of
+// matrices maps each matrix's rows (cut into parts of bounded nonzero count)
+// to parts whose window is the matrix's column range.
+template <int L, typename T, typename IT, typename PT>
+__device__ __forceinline__ void spmv_part_rows(const PT* __restrict__ indptr, const IT* __restrict__ indices,
+                                               const T* __restrict__ data, const T* __restrict__ xb,
+                                               int64_t xoff, T* __restrict__ y, int64_t row_begin,
+                                               int64_t row_end, int64_t nnz) {
+  using A = typename spmv_acc_of<T>::type;
+  constexpr int64_t per_wave = 64 / L;
+  constexpr int64_t per_pass = per_wave * kLdsxWaves;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int sub = lane / L;
+  const int l = lane % L;
+  for (int64_t base = row_begin; base < row_end; base += per_pass) {  // uniform over the workgroup
+    const int64_t r = base + wave * per_wave + sub;
+    const bool active = r < row_end;
+    int64_t b = 0, e = 0;
+    if (active) {
+      b = static_cast<int64_t>(indptr[r]);
+      e = static_cast<int64_t>(indptr[r + 1]);
+    }
+    const bool whole = __all(!active || e + 3 <= nnz);
+    A acc = A(0);
+    if (active) {
+      for (int64_t q = (b & ~int64_t(3)) + 4 * l; q < e; q += 4 * L) {
+        chunk4<IT> c;
+        chunk4<T> v;
+        if (whole || q + 4 <= nnz) {
+          c = load_chunk_nt(indices + q);
+          v = load_chunk_nt(data + q);
+        } else {
+          c = load_chunk_tail(indices, q, nnz);
+          v = load_chunk_tail(data, q, nnz);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const bool ok = q + j >= b && q + j < e;
+          const int64_t col = ok ? static_cast<int64_t>(c.v[j]) : xoff;
+          const A t = fmadd(static_cast<A>(v.v[j]), static_cast<A>(xb[col - xoff]), acc);
+          acc = ok ? t : acc;
+        }
+      }
+    }
+    acc = group_sum(acc, L);
+    if (active && l == 0) y[r] = static_cast<T>(acc);
+  }
+}
+
+template <typename T, typename IT, typename PT>
+__global__ __launch_bounds__(kLdsxThreads) void spmv_parts_kernel(const PT* __restrict__ indptr,
+                                                                  const IT* __restrict__ indices,
+                                                                  const T* __restrict__ data,
+                                                                  const T* __restrict__ x, T* __restrict__ y,
+                                                                  const int64_t* __restrict__ parts, int lanes_log2,
+                                                                  int64_t nnz) {
+  extern __shared__ __align__(16) unsigned char fsp_part_mem[];
+  T* xs = reinterpret_cast<T*>(fsp_part_mem);
+  const int64_t* pt = parts + 4 * static_cast<int64_t>(blockIdx.x);
+  const int64_t row_begin = pt[0], row_end = pt[1], lo = pt[2], span = pt[3] - pt[2];
+  const bool fits = span >= 0 && span * static_cast<int64_t>(sizeof(T)) <= kLdsxBytes;  // uniform
+  if (fits) {
+    for (int64_t c = threadIdx.x; c < span; c += kLdsxThreads) xs[c] = x[lo + c];
+  }
+  __syncthreads();
+#define FSP_PART_ROWS(LN)                                                                                   \
+  if (fits) {                                                                                             \
+    spmv_part_rows<LN, T, IT, PT>(indptr, indices, data, xs, lo, y, row_begin, row_end, nnz);            \
+  } else {                                                                                                \
+    spmv_part_rows<LN, T, IT, PT>(indptr, indices, data, x, 0, y, row_begin, row_end, nnz);              \
+  }
+  switch (lanes_log2) {  // uniform
+    case 2: FSP_PART_ROWS(4); break;
+    case 3: FSP_PART_ROWS(8); break;
+    default: FSP_PART_ROWS(16); break;
+  }
+#undef FSP_PART_ROWS
+}
+
 // Compute units of the current device (cached per device).
 static int device_cu_count() {
   static std::atomic<int> cache[64];
@@ -515,6 +599,38 @@ extern "C" int fsp_spmv_csr(int32_t value_type, int32_t index_type, int32_t indp
           return launch_spmv<T, IT, PT, QT, true>(indptr_dev, indices_dev, data_dev, perm_dev, x_dev, y_dev,
                                                   x_len, nnz, plan, s);
         });
+      });
+    });
+  });
+  return FSP_OK;
+}
+
+extern "C" int fsp_spmv_csr_parts(int32_t value_type, int32_t index_type, int32_t indptr_type,
+                                  const void* indptr_dev, const void* indices_dev, const void* data_dev,
+                                  const void* x_dev, void* y_dev, int64_t nnz, int64_t n_parts,
+                                  const int64_t* parts_dev, int32_t lanes, void* stream) {
+  if (nnz < 0 || n_parts < 0 || (lanes != 4 && lanes != 8 && lanes != 16)) {
+    return fail(FSP_ERR_ARG, "fsp_spmv_csr_parts: bad arguments (lanes must be 4, 8 or 16)");
+  }
+  if (n_parts == 0) return FSP_OK;
+  if (n_parts > 0x7fffffff || parts_dev == nullptr || indptr_dev == nullptr || y_dev == nullptr ||
+      (nnz > 0 && (indices_dev == nullptr || data_dev == nullptr || x_dev == nullptr))) {
+    return fail(FSP_ERR_ARG, "fsp_spmv_csr_parts: NULL array or too many parts");
+  }
+  const hipStream_t s = as_stream(stream);
+  const int ll = lanes == 4 ? 2 : (lanes == 8 ? 3 : 4);
+  FSP_DISPATCH_VALUE(value_type, T, {
+    FSP_DISPATCH_INDEX(index_type, IT, {
+      FSP_DISPATCH_INDEX(indptr_type, PT, {
+        if (reinterpret_cast<uintptr_t>(indices_dev) % (4 * sizeof(IT)) != 0 ||
+            reinterpret_cast<uintptr_t>(data_dev) % (4 * sizeof(T)) != 0) {
+          return fail(FSP_ERR_ARG, "fsp_spmv_csr_parts: indices and data must be aligned to 4 elements");
+        }
+        hipLaunchKernelGGL((spmv_parts_kernel<T, IT, PT>), dim3(static_cast<unsigned>(n_parts)), dim3(kLdsxThreads),
+                           static_cast<size_t>(kLdsxBytes), s, static_cast<const PT*>(indptr_dev),
+                           static_cast<const IT*>(indices_dev), static_cast<const T*>(data_dev),
+                           static_cast<const T*>(x_dev), static_cast<T*>(y_dev), parts_dev, ll, nnz);
+        return check_launch("spmv_parts");
       });
     });
   });

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSP_ABI_VERSION 8
+#define FSP_ABI_VERSION 9
 #define FSP_MAX_BINS 8
 
 typedef enum fsp_value_type {
@@ -159,6 +159,21 @@ int fsp_spmv_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
                  const void* data_dev, int32_t perm_type, const void* perm_dev,
                  const void* x_dev, void* y_dev, int64_t n_rows, int64_t nnz,
                  int64_t x_len, const fsp_csr_plan* plan, void* stream);
+
+/* Row-range SpMV for matrices whose rows fall into groups with narrow column
+ * ranges (a batch of matrices is the case it serves; no counterpart in the
+ * reference, which runs one matrix per call).  parts_dev[n_parts][4] holds
+ * (row_begin, row_end, col_lo, col_hi) per part: rows [row_begin, row_end)
+ * use only columns [col_lo, col_hi).  One 1024-thread workgroup per part
+ * copies x[col_lo, col_hi) into LDS when it holds at most 64 KiB (else it
+ * gathers from global memory) and runs the rows with `lanes` (4, 8 or 16)
+ * lanes per row, reading indices and data as 4-element chunks (both aligned
+ * to 4 elements).  Parts must not share rows; rows outside every part are not
+ * written.  Accumulation as fsp_spmv_csr; nnz = length of indices / data. */
+int fsp_spmv_csr_parts(int32_t value_type, int32_t index_type, int32_t indptr_type,
+                       const void* indptr_dev, const void* indices_dev, const void* data_dev,
+                       const void* x_dev, void* y_dev, int64_t nnz, int64_t n_parts,
+                       const int64_t* parts_dev, int32_t lanes, void* stream);
 
 /* ---- SpMM ----------------------------------------------------------------
  * C[n_rows, n_dense] = A B with B row-major (ldb >= n_dense) and C row-major

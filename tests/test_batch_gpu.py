@@ -177,7 +177,8 @@ def test_autotune_picks_and_caches(device, tmp_path, monkeypatch):
     p = b.params("spmv")
     from flagsparse_amd import _plan
 
-    cands = autotune.candidates("spmv", 1, 4, b.total_cols, _plan.chunk_aligned(b.indices, b.data))
+    cands = autotune.batch_spmv_candidates(4, b.total_cols, _plan.chunk_aligned(b.indices, b.data))
+    assert any(c.variant == 5 for c in cands)  # the packed arrays sit on 4-element boundaries
     assert p in cands
     assert len(b.tune_results) == len(cands)
     assert path.exists() and "spmv|float32" in path.read_text()
@@ -236,3 +237,39 @@ def test_tune_spmv_single_matrix(device):
     y = h.spmv(data, S.dense(x, device), config=p.spmv_config(), max_blocks=p.max_blocks(device),
                xcd_groups=p.xcd_groups, variant=p.variant)
     assert rel_err(y, O.spmv(ip, ix, v, x, (30000, 30000))) <= 1e-5
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
+@pytest.mark.parametrize("index_dtype", [torch.int32, torch.int64], ids=["i32", "i64"])
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+def test_batch_spmv_row_range_parts(device, dtype, index_dtype, lanes):
+    """Variant 5 (row-range parts, each matrix's column range of x in LDS):
+    the oracle per matrix, matrices cut into several parts (small part size)
+    or one, a matrix whose columns exceed the LDS window (global gathers),
+    bitwise equal to its per-matrix loop, and graph replays equal to eager."""
+    from flagsparse_amd.autotune import TuneParams
+
+    mats = _small_batch(1)
+    wide = S.random_csr(300, 20000, 0.01, seed=77)  # 20000 columns > the 64 KiB window for f32 / f64
+    mats.append(wide + ((300, 20000),))
+    b = _to_batch(mats, device, dtype, index_dtype)
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, b.total_cols)
+    xt = S.dense(x, device, dtype)
+    for part_nnz in (64, 1 << 17):
+        p = TuneParams(unroll=4, lanes_n=lanes, variant=5, part_nnz=part_nnz)
+        parts, start = b.parts(part_nnz)
+        assert int(parts[:, 1].sub(parts[:, 0]).sum()) == b.total_rows  # every row in exactly one part
+        y = b.spmv(xt, params=p)
+        for m, (ip, ix, v, shape) in enumerate(mats):
+            want = O.spmv(ip, ix, v, x[b.col_offsets[m]: b.col_offsets[m + 1]], shape)
+            got = y[b.row_offsets[m]: b.row_offsets[m + 1]]
+            assert rel_err(got, want) <= REL_TOL[dtype], f"matrix {m} part_nnz {part_nnz}"
+        assert torch.equal(b.spmv_per_matrix(xt, params=p), y)
+        yg = torch.empty_like(y)
+        g = b._capture(lambda: b.spmv(xt, out=yg, params=p))
+        for rep in range(3):
+            yg.zero_()
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(yg, y), f"replay {rep}"
