@@ -44,6 +44,30 @@ def _host_offsets(v, name, n=None):
     return a
 
 
+def parts_table(row_offsets, col_offsets, nnz_offsets, part_nnz):
+    """Row-range parts of a batch for fsp_spmv_csr_parts (host arrays).
+
+    Matrix m's rows are cut evenly into min(rows_m, ceil(nnz_m / part_nnz))
+    ranges (at least one for a matrix with rows); each part's window is the
+    matrix's column range.  Returns (table int64 [P, 4] of (row_begin,
+    row_end, col_lo, col_hi), start int64 [n_mats + 1]: matrix m owns parts
+    start[m] .. start[m+1]-1)."""
+    rows = np.asarray(row_offsets, dtype=np.int64)
+    cols = np.asarray(col_offsets, dtype=np.int64)
+    nnz = np.asarray(nnz_offsets, dtype=np.int64)
+    n_mats = rows.size - 1
+    n_rows = rows[1:] - rows[:-1]
+    n_parts = np.where(n_rows > 0, np.maximum(1, -(-(nnz[1:] - nnz[:-1]) // max(1, int(part_nnz)))), 0)
+    n_parts = np.minimum(n_parts, n_rows)
+    start = np.concatenate([[0], np.cumsum(n_parts)]).astype(np.int64)
+    owner = np.repeat(np.arange(n_mats, dtype=np.int64), n_parts)
+    k = np.arange(int(start[-1]), dtype=np.int64) - start[owner]
+    rb = rows[owner] + n_rows[owner] * k // n_parts[owner]
+    re = rows[owner] + n_rows[owner] * (k + 1) // n_parts[owner]
+    table = np.stack([rb, re, cols[owner], cols[owner + 1]], 1).astype(np.int64).reshape(-1, 4)
+    return table, start
+
+
 class CsrBatch:
     """A packed batch of CSR matrices (see the module docstring)."""
 
@@ -219,18 +243,7 @@ class CsrBatch:
         hit = self._parts.get(key)
         if hit is not None:
             return hit
-        rows = np.asarray(self.row_offsets, dtype=np.int64)
-        cols = np.asarray(self.col_offsets, dtype=np.int64)
-        nnz = np.asarray(self.nnz_offsets, dtype=np.int64)
-        n_rows = rows[1:] - rows[:-1]
-        n_parts = np.where(n_rows > 0, np.maximum(1, -(-(nnz[1:] - nnz[:-1]) // max(1, key))), 0)
-        n_parts = np.minimum(n_parts, n_rows)
-        start = np.concatenate([[0], np.cumsum(n_parts)])
-        owner = np.repeat(np.arange(self.n_mats, dtype=np.int64), n_parts)
-        k = np.arange(int(start[-1]), dtype=np.int64) - start[owner]
-        rb = rows[owner] + n_rows[owner] * k // n_parts[owner]
-        re = rows[owner] + n_rows[owner] * (k + 1) // n_parts[owner]
-        table = np.stack([rb, re, cols[owner], cols[owner + 1]], 1).astype(np.int64)
+        table, start = parts_table(self.row_offsets, self.col_offsets, self.nnz_offsets, key)
         dev = torch.from_numpy(np.ascontiguousarray(table)).to(self.device)
         self._parts[key] = (dev, start)
         return self._parts[key]

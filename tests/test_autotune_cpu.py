@@ -140,3 +140,35 @@ def test_log_uniform_rows_and_lengths_distribution():
     lens = np.diff(ip.numpy().astype(np.int64))
     lens = lens[lens >= 0]  # drop the jumps back to 0 between matrices
     assert abs(lens.mean() - 34) < 0.2
+
+
+def test_batch_parts_table_covers_rows_once():
+    import numpy as np
+
+    from flagsparse_amd.batch import parts_table
+
+    rng = np.random.default_rng(0)
+    for trial in range(50):
+        n_mats = int(rng.integers(1, 40))
+        n_rows = rng.integers(0, 300, n_mats) * (rng.random(n_mats) > 0.15)  # some empty matrices
+        n_cols = rng.integers(0, 500, n_mats)
+        nnz_m = np.where(n_rows > 0, rng.integers(0, 20000, n_mats), 0)
+        rows = np.concatenate([[0], np.cumsum(n_rows)])
+        cols = np.concatenate([[0], np.cumsum(n_cols)])
+        nnz = np.concatenate([[0], np.cumsum(nnz_m)])
+        part_nnz = int(rng.choice([1, 64, 1000, 1 << 15]))
+        table, start = parts_table(rows, cols, nnz, part_nnz)
+        assert table.shape == (start[-1], 4) and start.shape == (n_mats + 1,)
+        covered = np.zeros(rows[-1], dtype=np.int64)
+        for m in range(n_mats):
+            own = table[start[m]:start[m + 1]]
+            if n_rows[m] == 0:
+                assert own.shape[0] == 0
+                continue
+            assert own.shape[0] == min(n_rows[m], max(1, -(-nnz_m[m] // part_nnz)))
+            assert own[0, 0] == rows[m] and own[-1, 1] == rows[m + 1]
+            assert (own[:, 1] > own[:, 0]).all() and (own[1:, 0] == own[:-1, 1]).all()  # contiguous, non-empty
+            assert (own[:, 2] == cols[m]).all() and (own[:, 3] == cols[m + 1]).all()
+            for rb, re, _, _ in own:
+                covered[rb:re] += 1
+        assert (covered == 1).all()
