@@ -160,10 +160,33 @@ def bench_spmv_uniform(dev, steps, warmup):
                                                xcd_groups=p.xcd_groups, variant=p.variant), dev, 32, 5)
         if best is None or ms < best[0]:
             best = (ms, p)
-    return {"ms": wall_ms / (steps * 5), "event_ms_median": statistics.median(ev), "bytes": nbytes,
-            "graph_ms_per_call": graph_ms, "gbs_cache_resident": nbytes / (graph_ms * 1e-3) / 1e9,
-            "best_graph_ms_per_call": best[0], "best_params": best[1].__dict__,
-            "best_gbs_cache_resident": nbytes / (best[0] * 1e-3) / 1e9}
+    out = {"ms": wall_ms / (steps * 5), "event_ms_median": statistics.median(ev), "bytes": nbytes,
+           "graph_ms_per_call": graph_ms, "gbs_cache_resident": nbytes / (graph_ms * 1e-3) / 1e9,
+           "best_graph_ms_per_call": best[0], "best_params": best[1].__dict__,
+           "best_gbs_cache_resident": nbytes / (best[0] * 1e-3) / 1e9}
+    out["replicas_256"] = bench_spmv_replicas(dev, data, idx, ptr, xt, nbytes)
+    return out
+
+
+def bench_spmv_replicas(dev, data, idx, ptr, xt, nbytes, n_rep=256):
+    """The 16384^2 x 16 SpMV over 256 independent in-memory replicas (587 MB,
+    beyond the 256 MB Infinity Cache), all products in one batched launch
+    (graph replay): the HBM-resident rate per product."""
+    from flagsparse_amd.batch import CsrBatch  # noqa: PLC0415
+
+    b = CsrBatch.from_matrices([(data, idx, ptr, (16384, 16384))] * n_rep)
+    x = xt.repeat(n_rep)
+    y = torch.empty(b.total_rows, dtype=torch.float32, device=dev)
+    p = b.params("spmv")
+    g = b.graph_spmv(x, y)
+    stream = torch.cuda.current_stream(dev)
+    _, ev = time_steps(g.replay, 20, 3, stream)
+    med = statistics.median(ev)
+    ok = torch.equal(y[:16384], b.spmv(x, params=p)[:16384])
+    del b, x, y, g
+    return {"replicas": n_rep, "bytes": nbytes * n_rep, "graph_event_ms_median": round(med, 5),
+            "us_per_product": round(med * 1e3 / n_rep, 4), "gbs": round(nbytes * n_rep / (med * 1e-3) / 1e9, 2),
+            "tuned": p.__dict__, "replay_equals_eager": bool(ok)}
 
 
 def bench_spmm_tuned(dev, steps, warmup, inputs):
