@@ -33,7 +33,7 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
   using O = typename addr<OFF32>::off_t;  // element / byte offsets
   using P = typename std::conditional<OFF32, int32_t, int64_t>::type;  // nonzero positions
   const int k = find_bin(tab, blockIdx.x);
-  const launch_bin& lb = tab.b[k];
+  const launch_bin lb = tab.b[k];  // a copy: the fields stay in registers
   const int s_log2 = lb.lanes_log2;
   const int G = 1 << g_log2;
   const int S = 1 << s_log2;
