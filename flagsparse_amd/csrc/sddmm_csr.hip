@@ -48,7 +48,7 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void sddmm_bins_kernel(
   using P = typename std::conditional<OFF32, int32_t, int64_t>::type;
   using V = vec_t<T, VEC>;
   const int kb = find_bin(tab, blockIdx.x);
-  const launch_bin& lb = tab.b[kb];
+  const launch_bin lb = tab.b[kb];  // a copy: the fields stay in registers
   const int s_log2 = lb.lanes_log2;
   const int G = 1 << g_log2;
   const int S = 1 << s_log2;
