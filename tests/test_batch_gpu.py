@@ -239,7 +239,8 @@ def test_tune_spmv_single_matrix(device):
     assert rel_err(y, O.spmv(ip, ix, v, x, (30000, 30000))) <= 1e-5
 
 
-@pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.bfloat16, torch.float16],
+                         ids=["f32", "f64", "bf16", "f16"])
 @pytest.mark.parametrize("index_dtype", [torch.int32, torch.int64], ids=["i32", "i64"])
 @pytest.mark.parametrize("lanes", [4, 8, 16])
 def test_batch_spmv_row_range_parts(device, dtype, index_dtype, lanes):
@@ -261,8 +262,9 @@ def test_batch_spmv_row_range_parts(device, dtype, index_dtype, lanes):
         parts, start = b.parts(part_nnz)
         assert int(parts[:, 1].sub(parts[:, 0]).sum()) == b.total_rows  # every row in exactly one part
         y = b.spmv(xt, params=p)
+        xr = _rounded(x, dtype)
         for m, (ip, ix, v, shape) in enumerate(mats):
-            want = O.spmv(ip, ix, v, x[b.col_offsets[m]: b.col_offsets[m + 1]], shape)
+            want = O.spmv(ip, ix, _rounded(v, dtype), xr[b.col_offsets[m]: b.col_offsets[m + 1]], shape)
             got = y[b.row_offsets[m]: b.row_offsets[m + 1]]
             assert rel_err(got, want) <= REL_TOL[dtype], f"matrix {m} part_nnz {part_nnz}"
         assert torch.equal(b.spmv_per_matrix(xt, params=p), y)
@@ -273,3 +275,28 @@ def test_batch_spmv_row_range_parts(device, dtype, index_dtype, lanes):
             g.replay()
             torch.cuda.synchronize()
             assert torch.equal(yg, y), f"replay {rep}"
+
+
+def _rounded(a, dtype):
+    """Values as the kernel sees them (rounded to dtype), in float64."""
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dtype).to(torch.float64).numpy()
+
+
+def test_spmv_parts_entry_validation(device):
+    from flagsparse_amd import _lib
+
+    lib = _lib.load()
+    one = torch.zeros(8, dtype=torch.int32, device=device)
+    parts = torch.zeros(1, 4, dtype=torch.int64, device=device)
+    y = torch.zeros(4, device=device)
+    vals = torch.zeros(9, device=device)
+
+    def call(data_ptr, n_parts, lanes):
+        return lib.fsp_spmv_csr_parts(0, 0, 0, one.data_ptr(), one.data_ptr(), data_ptr, y.data_ptr(),
+                                      y.data_ptr(), 4, n_parts, parts.data_ptr(), lanes, _lib.stream_ptr(device))
+
+    with pytest.raises(RuntimeError, match="lanes"):
+        _lib.check(call(y.data_ptr(), 1, 5), "fsp_spmv_csr_parts")
+    with pytest.raises(RuntimeError, match="aligned"):
+        _lib.check(call(vals[1:].data_ptr(), 1, 4), "fsp_spmv_csr_parts")
+    assert call(y.data_ptr(), 0, 4) == 0  # no parts: nothing to launch
