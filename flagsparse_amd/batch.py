@@ -13,12 +13,13 @@ row offsets.
 At construction the batch is packed once on the GPU into its block-diagonal
 CSR (global row pointer and column ids; the values are shared, so they may
 change between calls) and validated.  ``spmv`` / ``spmm`` are then ONE
-launch of the row-bin kernels over that matrix -- persistent when the
-autotuner picks a capped grid -- and ``graph_spmv`` / ``graph_spmm`` capture
-it in a HIP graph.  Each row is reduced in the same order as in a launch
-over its own matrix with the same parameters, so the batched result is
-bitwise equal to ``spmv_per_matrix`` / ``spmm_per_matrix``, the per-matrix
-launch loop kept as the comparator.
+launch over that matrix -- the row-bin kernels (persistent when the
+autotuner picks a capped grid) or, for the SpMV, the row-range parts kernel
+that keeps each matrix's column range of x in LDS -- and ``graph_spmv`` /
+``graph_spmm`` capture it in a HIP graph.  Each row is reduced in the same
+order as in a launch of the same kernel over its own matrix, so the batched
+result is bitwise equal to ``spmv_per_matrix`` / ``spmm_per_matrix``, the
+per-matrix launch loop kept as the comparator.
 """
 
 import numpy as np

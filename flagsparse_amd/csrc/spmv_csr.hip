@@ -1,14 +1,22 @@
 // CSR SpMV y = A x on gfx950.
 //
 // Rows are binned by length; a bin with lanes L puts 64/L rows on one wave and
-// splits each row's nonzeros over L lanes (lane l takes nonzeros l, l+L, ...),
-// UNROLL independent (index, value, x) loads in flight per lane, then a DPP
-// group sum.  Software-pipelined like the SpMM: while a trip's x gathers are
-// in flight a lane loads the column ids / values of its next trip -- of the
-// same row, or the first trip of its next row (whose row pointer was loaded
-// one row earlier).  Rows longer than the last bin are cut into segments; one wave
-// reduces each segment into a partial and a second pass adds a row's partials
-// in segment order (no atomics: the result is deterministic for a plan).
+// splits each row's nonzeros over L lanes, then a DPP group sum.  Rows longer
+// than the last bin are cut into segments; one wave reduces each segment into
+// a partial and a second pass adds a row's partials in segment order (no
+// atomics: the result is deterministic for a plan).  One fused launch runs
+// every bin and the segments with the loop plan.variant selects:
+//   0  spmv_bins_kernel: lane l takes nonzeros l, l+L, ..., UNROLL loads in
+//      flight, software-pipelined (the next trip's column ids / values load
+//      while this trip's x gathers are in flight);
+//   1  spmv_simple_kernel: the same order, plain loop;
+//   2  spmv_simple_kernel and 3 spmv_bins_kernel with x staged in LDS by
+//      1024-thread workgroups;
+//   4  spmv_vec_kernel: the index / value streams read as 4-element chunks,
+//      x in LDS.
+// Variants 0-3 are bitwise equal.  spmv_parts_kernel (fsp_spmv_csr_parts)
+// runs row-range parts, each with its own column window of x in LDS (the
+// batched SpMV).
 #include <atomic>
 
 #include "fsp_device.hpp"
