@@ -309,6 +309,10 @@ class CsrBatch:
     def spmv_per_matrix(self, x, out=None, data=None, params=None):
         """The same product as spmv, one launch per matrix (the comparator)."""
         data = self.data if data is None else data
+        self._check_dense(x, self.total_cols, "x")
+        if x.ndim != 1 or x.dtype != data.dtype:
+            raise ValueError("x must be 1D with the values' dtype")
+        x = x.contiguous()
         p = params or self.params("spmv", 1, data.dtype)
         y = out if out is not None else torch.empty(self.total_rows, dtype=data.dtype, device=self.device)
         if p.variant == 5:  # each matrix's own parts, one launch per matrix
@@ -324,7 +328,12 @@ class CsrBatch:
         return y
 
     def spmm_per_matrix(self, B, out=None, data=None, params=None):
+        """The same product as spmm, one launch per matrix (the comparator)."""
         data = self.data if data is None else data
+        self._check_dense(B, self.total_cols, "B")
+        if B.ndim != 2 or B.dtype != data.dtype:
+            raise ValueError("B must be 2D with the values' dtype")
+        B = B.contiguous()
         n_dense = int(B.shape[1])
         p = params or self.params("spmm", n_dense, data.dtype)
         lanes = self._lanes(p, n_dense, data.element_size())

@@ -300,3 +300,17 @@ def test_spmv_parts_entry_validation(device):
     with pytest.raises(RuntimeError, match="aligned"):
         _lib.check(call(vals[1:].data_ptr(), 1, 4), "fsp_spmv_csr_parts")
     assert call(y.data_ptr(), 0, 4) == 0  # no parts: nothing to launch
+
+
+def test_per_matrix_loops_take_strided_operands(device):
+    from flagsparse_amd.autotune import TuneParams
+
+    b = _to_batch(_small_batch(3), device)
+    xs = torch.rand(2 * b.total_cols, device=device)[::2]  # strided view
+    Bs = torch.rand(b.total_cols, 24, device=device)[:, ::2]  # strided columns
+    for p in (TuneParams(), TuneParams(unroll=4, lanes_n=8, variant=5, part_nnz=256)):
+        assert torch.equal(b.spmv_per_matrix(xs, params=p), b.spmv(xs.contiguous(), params=p))
+    pm = TuneParams()
+    assert torch.equal(b.spmm_per_matrix(Bs, params=pm), b.spmm(Bs.contiguous(), params=pm))
+    with pytest.raises(ValueError, match="x.shape"):
+        b.spmv_per_matrix(xs[1:])
