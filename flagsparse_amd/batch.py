@@ -219,12 +219,22 @@ class CsrBatch:
         if t.shape[0] != n:
             raise ValueError(f"{what}.shape[0] must be {n}, got {t.shape[0]}")
 
+    def _check_out(self, out, shape, dtype):
+        """`out` (written in place by the kernels): a contiguous tensor of the
+        result's shape and dtype on the batch's device, or None."""
+        if out is None:
+            return
+        if (not torch.is_tensor(out) or out.device != self.device or tuple(out.shape) != tuple(shape)
+                or out.dtype != dtype or not out.is_contiguous()):
+            raise ValueError(f"out must be a contiguous {dtype} tensor of shape {tuple(shape)} on the batch's device")
+
     def spmv(self, x, out=None, data=None, params=None):
         """y (packed by rows) = blockdiag(A_m) x (packed by columns): one launch."""
         data = self.data if data is None else data
         self._check_dense(x, self.total_cols, "x")
         if x.ndim != 1 or x.dtype != data.dtype:
             raise ValueError("x must be 1D with the values' dtype")
+        self._check_out(out, (self.total_rows,), data.dtype)
         p = params or self.params("spmv", 1, data.dtype)
         if p.variant == 5:
             y = out if out is not None else torch.empty(self.total_rows, dtype=data.dtype, device=self.device)
@@ -268,6 +278,7 @@ class CsrBatch:
         if B.ndim != 2 or B.dtype != data.dtype:
             raise ValueError("B must be 2D with the values' dtype")
         n_dense = int(B.shape[1])
+        self._check_out(out, (self.total_rows, n_dense), data.dtype)
         p = params or self.params("spmm", n_dense, data.dtype)
         lanes = self._lanes(p, n_dense, data.element_size())
         return self.handle.spmm(data, B.contiguous(), out=out, lanes_n=lanes, config=p.spmm_config(lanes),
@@ -313,6 +324,7 @@ class CsrBatch:
         if x.ndim != 1 or x.dtype != data.dtype:
             raise ValueError("x must be 1D with the values' dtype")
         x = x.contiguous()
+        self._check_out(out, (self.total_rows,), data.dtype)
         p = params or self.params("spmv", 1, data.dtype)
         y = out if out is not None else torch.empty(self.total_rows, dtype=data.dtype, device=self.device)
         if p.variant == 5:  # each matrix's own parts, one launch per matrix
@@ -335,6 +347,7 @@ class CsrBatch:
             raise ValueError("B must be 2D with the values' dtype")
         B = B.contiguous()
         n_dense = int(B.shape[1])
+        self._check_out(out, (self.total_rows, n_dense), data.dtype)
         p = params or self.params("spmm", n_dense, data.dtype)
         lanes = self._lanes(p, n_dense, data.element_size())
         cfg = p.spmm_config(lanes)

@@ -314,3 +314,8 @@ def test_per_matrix_loops_take_strided_operands(device):
     assert torch.equal(b.spmm_per_matrix(Bs, params=pm), b.spmm(Bs.contiguous(), params=pm))
     with pytest.raises(ValueError, match="x.shape"):
         b.spmv_per_matrix(xs[1:])
+    y_strided = torch.empty(2 * b.total_rows, device=device)[::2]
+    with pytest.raises(ValueError, match="contiguous"):
+        b.spmv(xs.contiguous(), out=y_strided)
+    with pytest.raises(ValueError, match="contiguous"):
+        b.spmm(Bs.contiguous(), out=torch.empty(b.total_rows, 13, device=device))
