@@ -61,6 +61,11 @@ class _SpmmCsrFunction(torch.autograd.Function):
             raise TypeError("B dtype must match values dtype")
         if B.ndim != 2 or B.shape[0] != matrix.shape[1]:
             raise ValueError(f"B must be 2D with B.shape[0] == n_cols={matrix.shape[1]}")
+        if values.ndim != 1 or values.numel() != matrix.handle.nnz:
+            raise ValueError(f"values must be 1D with one entry per nonzero (nnz={matrix.handle.nnz})")
+        device = matrix.indices.device
+        if not values.is_cuda or not B.is_cuda or values.device != device or B.device != device:
+            raise ValueError("values and B must be CUDA tensors on the matrix's device")
         values_c = values.contiguous()
         B_c = B.contiguous()
         ctx.matrix = matrix

@@ -213,6 +213,19 @@ class CsrBatch:
         return p.lanes_n or _plan.lanes_for_dense(n_dense, elem)
 
     # ------------------------------------------------------------ kernels
+    def _values(self, data):
+        """The values a call runs with: the batch's own (None) or a replacement
+        of the same pattern, one per nonzero on the batch's device."""
+        if data is None:
+            return self.data
+        if not torch.is_tensor(data) or not data.is_cuda or data.device != self.device:
+            raise ValueError("data must be a CUDA tensor on the batch's device")
+        if data.ndim != 1 or data.numel() != self.nnz:
+            raise ValueError(f"data must be 1D with nnz={self.nnz} entries, got shape {tuple(data.shape)}")
+        if data.dtype not in _VALUE_DTYPES:
+            raise TypeError("data dtype must be one of: float16, bfloat16, float32, float64")
+        return data.contiguous()
+
     def _check_dense(self, t, n, what):
         if not torch.is_tensor(t) or not t.is_cuda or t.device != self.device:
             raise ValueError(f"{what} must be a CUDA tensor on the batch's device")
@@ -230,7 +243,7 @@ class CsrBatch:
 
     def spmv(self, x, out=None, data=None, params=None):
         """y (packed by rows) = blockdiag(A_m) x (packed by columns): one launch."""
-        data = self.data if data is None else data
+        data = self._values(data)
         self._check_dense(x, self.total_cols, "x")
         if x.ndim != 1 or x.dtype != data.dtype:
             raise ValueError("x must be 1D with the values' dtype")
@@ -273,7 +286,7 @@ class CsrBatch:
 
     def spmm(self, B, out=None, data=None, params=None):
         """C (packed by rows) = blockdiag(A_m) B (packed by rows): one launch."""
-        data = self.data if data is None else data
+        data = self._values(data)
         self._check_dense(B, self.total_cols, "B")
         if B.ndim != 2 or B.dtype != data.dtype:
             raise ValueError("B must be 2D with the values' dtype")
@@ -295,9 +308,14 @@ class CsrBatch:
 
     def graph_spmv(self, x, y, data=None):
         """HIP graph of spmv(x, out=y); replay() reruns it on the current x."""
+        if y is None:
+            raise ValueError("graph_spmv needs the output tensor y: the graph writes into it")
         return self._capture(lambda: self.spmv(x, out=y, data=data))
 
     def graph_spmm(self, B, C, data=None):
+        """HIP graph of spmm(B, out=C); replay() reruns it on the current B."""
+        if C is None:
+            raise ValueError("graph_spmm needs the output tensor C: the graph writes into it")
         return self._capture(lambda: self.spmm(B, out=C, data=data))
 
     # ------------------------------------------- per-matrix comparator
@@ -319,7 +337,7 @@ class CsrBatch:
 
     def spmv_per_matrix(self, x, out=None, data=None, params=None):
         """The same product as spmv, one launch per matrix (the comparator)."""
-        data = self.data if data is None else data
+        data = self._values(data)
         self._check_dense(x, self.total_cols, "x")
         if x.ndim != 1 or x.dtype != data.dtype:
             raise ValueError("x must be 1D with the values' dtype")
@@ -341,7 +359,7 @@ class CsrBatch:
 
     def spmm_per_matrix(self, B, out=None, data=None, params=None):
         """The same product as spmm, one launch per matrix (the comparator)."""
-        data = self.data if data is None else data
+        data = self._values(data)
         self._check_dense(B, self.total_cols, "B")
         if B.ndim != 2 or B.dtype != data.dtype:
             raise ValueError("B must be 2D with the values' dtype")

@@ -116,6 +116,8 @@ def _validate_sddmm_dense_inputs(data, prepared, x, y):
         raise TypeError("y dtype must match x dtype")
     if data is not None and data.dtype != x.dtype:
         raise TypeError("data dtype must match x/y dtype")
+    if data is not None and (not torch.is_tensor(data) or not data.is_cuda or data.device != x.device):
+        raise ValueError("data must be a CUDA tensor on the same device as x")
     if x.shape[0] != prepared.n_rows:
         raise ValueError(f"x.shape[0] must be n_rows={prepared.n_rows}, got {x.shape[0]}")
     if y.shape[0] != prepared.n_cols:
@@ -159,7 +161,11 @@ def _run_sddmm_prepared(prepared, x, y, data, alpha, beta, out, allow_fallback=F
     }
     if prepared.nnz == 0:
         return out, meta
-    prepared._handle.sddmm(x, y, alpha=alpha, beta=beta, data_in=data, out=out)
+    # the kernel writes nnz consecutive elements: a strided `out` receives a copy
+    target = out if out.is_contiguous() else torch.empty(prepared.nnz, dtype=out.dtype, device=out.device)
+    prepared._handle.sddmm(x, y, alpha=alpha, beta=beta, data_in=data, out=target)
+    if target is not out:
+        out.copy_(target)
     return out, meta
 
 

@@ -76,3 +76,18 @@ def test_only_requested_gradients(device):
     vals = data.clone().requires_grad_(True)
     fs.spmm_csr_autograd(vals, torch.randn(300, 16, device=device), mat2).sum().backward()
     assert vals.grad is not None and mat2.handle._csc is None  # no CSC needed without dB
+
+
+def test_rejects_mismatched_values_and_devices(device):
+    ip, ix, v = _graph(200, seed=2)
+    data, idx, ptr = device_csr(ip, ix, v, device, torch.float32)
+    mat = fs.CsrMatrix(idx, ptr, (200, 200))
+    B = torch.randn(200, 8, device=device)
+    with pytest.raises(ValueError, match="one entry per nonzero"):
+        fs.spmm_csr_autograd(data[:-1], B, mat)
+    with pytest.raises(ValueError, match="one entry per nonzero"):
+        fs.spmm_csr_autograd(data.reshape(1, -1), B, mat)
+    with pytest.raises(ValueError, match="matrix's device"):
+        fs.spmm_csr_autograd(data, B.cpu(), mat)
+    with pytest.raises(ValueError, match="matrix's device"):
+        fs.spmm_csr_autograd(data.cpu(), B, mat)

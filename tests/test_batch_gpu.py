@@ -167,6 +167,24 @@ def test_batch_validation(device):
         CsrBatch(data, indices, indptr, ro[::-1].copy(), co, no)
     with pytest.raises(ValueError, match="x.shape"):
         good.spmv(torch.zeros(good.total_cols + 1, device=device))
+    # replacement values: one per nonzero, on the batch's device, a kernel dtype
+    x = torch.zeros(good.total_cols, device=device)
+    B = torch.zeros((good.total_cols, 4), device=device)
+    for call in (lambda d: good.spmv(x, data=d), lambda d: good.spmm(B, data=d),
+                 lambda d: good.spmv_per_matrix(x, data=d), lambda d: good.spmm_per_matrix(B, data=d)):
+        with pytest.raises(ValueError, match="nnz="):
+            call(data[:-1])
+        with pytest.raises(ValueError, match="nnz="):
+            call(data.reshape(1, -1))
+        with pytest.raises(ValueError, match="batch's device"):
+            call(data.cpu())
+        with pytest.raises(TypeError, match="data dtype"):
+            call(torch.zeros(good.nnz, dtype=torch.int32, device=device))
+    # strided replacement values are made contiguous and give the same result
+    strided = torch.stack([data, data], 1)[:, 0]
+    assert torch.equal(good.spmv(x + 1, data=strided), good.spmv(x + 1))
+    with pytest.raises(ValueError, match="output tensor"):
+        good.graph_spmv(x, None)
 
 
 def test_autotune_picks_and_caches(device, tmp_path, monkeypatch):

@@ -111,3 +111,21 @@ def test_sddmm_half_storage_via_handle(device, dtype):
     got = h.sddmm(torch.tensor(X, dtype=dtype, device=device), torch.tensor(Y, dtype=dtype, device=device))
     assert got.dtype == dtype
     assert_rel(got, oracle.sddmm(ip, ix, X, Y), dtype)
+
+
+def test_sddmm_strided_out_and_host_data(device):
+    """A strided `out` receives the result without touching the elements
+    between its own; values on the host are rejected before any launch."""
+    ip, ix, v = synthetic.random_csr(40, 30, 0.2, seed=21)
+    data, idx, ptr = device_csr(ip, ix, v, device)
+    rng = np.random.default_rng(22)
+    X = torch.tensor(rng.standard_normal((40, 8)), dtype=torch.float32, device=device)
+    Y = torch.tensor(rng.standard_normal((30, 8)), dtype=torch.float32, device=device)
+    want = fs.flagsparse_sddmm_csr(data=data, indices=idx, indptr=ptr, x=X, y=Y, shape=(40, 30), beta=0.5)
+    buf = torch.full((2 * data.numel(),), 7.0, device=device)
+    got = fs.flagsparse_sddmm_csr(data=data, indices=idx, indptr=ptr, x=X, y=Y, shape=(40, 30), beta=0.5,
+                                  out=buf[::2])
+    assert got.data_ptr() == buf.data_ptr() and torch.equal(buf[::2], want)
+    assert torch.all(buf[1::2] == 7.0)
+    with pytest.raises(ValueError, match="data must be a CUDA tensor"):
+        fs.flagsparse_sddmm_csr(data=data.cpu(), indices=idx, indptr=ptr, x=X, y=Y, shape=(40, 30), beta=0.5)
