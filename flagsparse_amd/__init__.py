@@ -22,6 +22,25 @@ from ._common import (
     error_metrics,
     validation_metrics,
 )
+from .alpha_spmm_alg1 import (
+    PreparedAlphaSpmmAlg1,
+    alpha_spmm_alg1_tle_opt2_unavailable_reason,
+    alpha_spmm_alg1_tle_opt_unavailable_reason,
+    alpha_spmm_alg1_tle_unavailable_reason,
+    build_alpha_spmm_alg1_tle_opt2_meta,
+    build_alpha_spmm_alg1_tle_opt_meta,
+    flagsparse_alpha_spmm_alg1,
+    flagsparse_alpha_spmm_alg1_tle,
+    flagsparse_alpha_spmm_alg1_tle_opt,
+    flagsparse_alpha_spmm_alg1_tle_opt2,
+    is_alpha_spmm_alg1_tle_available,
+    is_alpha_spmm_alg1_tle_opt2_available,
+    is_alpha_spmm_alg1_tle_opt_available,
+    prepare_alpha_spmm_alg1,
+    prepare_alpha_spmm_alg1_tle,
+    prepare_alpha_spmm_alg1_tle_opt,
+    prepare_alpha_spmm_alg1_tle_opt2,
+)
 from .benchmarks import benchmark_sddmm_case, benchmark_spmm_case, benchmark_spmv_case
 from .autograd import CsrMatrix, flagsparse_spmm_csr_autograd, spmm_csr_autograd
 from .bsr import csr_to_bsr, flagsparse_spmm_bsr, quantize_bsr_fp8
@@ -57,6 +76,23 @@ from .spmv_csr import PreparedCsrSpmv, flagsparse_spmv_csr, prepare_spmv_csr
 __version__ = "0.1.0"
 
 __all__ = [
+    "PreparedAlphaSpmmAlg1",
+    "alpha_spmm_alg1_tle_opt2_unavailable_reason",
+    "alpha_spmm_alg1_tle_opt_unavailable_reason",
+    "alpha_spmm_alg1_tle_unavailable_reason",
+    "build_alpha_spmm_alg1_tle_opt2_meta",
+    "build_alpha_spmm_alg1_tle_opt_meta",
+    "flagsparse_alpha_spmm_alg1",
+    "flagsparse_alpha_spmm_alg1_tle",
+    "flagsparse_alpha_spmm_alg1_tle_opt",
+    "flagsparse_alpha_spmm_alg1_tle_opt2",
+    "is_alpha_spmm_alg1_tle_available",
+    "is_alpha_spmm_alg1_tle_opt2_available",
+    "is_alpha_spmm_alg1_tle_opt_available",
+    "prepare_alpha_spmm_alg1",
+    "prepare_alpha_spmm_alg1_tle",
+    "prepare_alpha_spmm_alg1_tle_opt",
+    "prepare_alpha_spmm_alg1_tle_opt2",
     "benchmark_spmv_case",
     "benchmark_spmm_case",
     "benchmark_sddmm_case",
