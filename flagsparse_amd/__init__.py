@@ -70,12 +70,26 @@ from .spmm_csr import (
 )
 from .mtx import load_mtx_to_csr_torch
 from .spmv_bsr import PreparedBsrSpmv, flagsparse_spmv_bsr, prepare_spmv_bsr
-from .spmv_coo import coo_to_csr_for_spmv, flagsparse_spmv_coo_tocsr, prepare_spmv_coo_tocsr
+from .spmv_coo import (
+    PreparedCoo,
+    coo_to_csr_for_spmv,
+    flagsparse_spmv_coo,
+    flagsparse_spmv_coo_tocsr,
+    prepare_spmv_coo,
+    prepare_spmv_coo_tocsr,
+)
 from .spmv_csr import PreparedCsrSpmv, flagsparse_spmv_csr, prepare_spmv_csr
+from .spmv_csc import PreparedCscSpmv, flagsparse_spmv_csc, prepare_spmv_csc
 
 __version__ = "0.1.0"
 
 __all__ = [
+    "PreparedCoo",
+    "flagsparse_spmv_coo",
+    "prepare_spmv_coo",
+    "PreparedCscSpmv",
+    "flagsparse_spmv_csc",
+    "prepare_spmv_csc",
     "PreparedAlphaSpmmAlg1",
     "alpha_spmm_alg1_tle_opt2_unavailable_reason",
     "alpha_spmm_alg1_tle_opt_unavailable_reason",
