@@ -41,7 +41,13 @@ from .alpha_spmm_alg1 import (
     prepare_alpha_spmm_alg1_tle_opt,
     prepare_alpha_spmm_alg1_tle_opt2,
 )
-from .benchmarks import benchmark_sddmm_case, benchmark_spmm_case, benchmark_spmv_case
+from .benchmarks import (
+    benchmark_sddmm_case,
+    benchmark_spmm_case,
+    benchmark_spmm_opt_alg2_case,
+    benchmark_spmm_opt_case,
+    benchmark_spmv_case,
+)
 from .autograd import CsrMatrix, flagsparse_spmm_csr_autograd, spmm_csr_autograd
 from .bsr import csr_to_bsr, flagsparse_spmm_bsr, quantize_bsr_fp8
 from .sddmm_csr import SDDMMPrepared, flagsparse_sddmm_csr, prepare_sddmm_csr
@@ -84,6 +90,8 @@ from .spmv_csc import PreparedCscSpmv, flagsparse_spmv_csc, prepare_spmv_csc
 __version__ = "0.1.0"
 
 __all__ = [
+    "benchmark_spmm_opt_case",
+    "benchmark_spmm_opt_alg2_case",
     "PreparedCoo",
     "flagsparse_spmv_coo",
     "prepare_spmv_coo",

@@ -291,3 +291,131 @@ def benchmark_sddmm_case(n_rows=1024, n_cols=1024, nnz=16384, k_dim=64, value_dt
         "backend_status": {"cusparse_unavailable_reason": _VENDOR_NOTE if run_cusparse else None},
         "samples": {"triton": ours, "pytorch": ref, "reference": golden},
     }
+
+
+# ------------------------------------------------------- SpMM opt / alg2 cases
+# (atol, rtol) of the reference's SpMM reference checks (spmm_csr.py:74-83)
+_OPT_TOLERANCE = {
+    torch.float16: (1e-3, 2e-3),
+    torch.bfloat16: (0.016, 1e-1),
+    torch.float32: (1.3e-6, 1e-3),
+    torch.float64: (1e-7, 1e-5),
+}
+
+
+def _opt_reference_error(candidate, reference, value_dtype):
+    """max |c - r| / (atol + rtol |r|): the opt cases' error figure
+    (spmm_csr.py:4304-4310, spmm_csr_opt_alg2.py:1243-1249); <= 1 passes."""
+    atol, rtol = _OPT_TOLERANCE.get(value_dtype, (1e-6, 1e-5))
+    if candidate.numel() == 0:
+        return 0.0
+    diff = torch.abs(candidate.to(torch.float64) - reference.to(torch.float64))
+    return float(torch.max(diff / (atol + rtol * torch.abs(reference.to(torch.float64)))).item())
+
+
+def _opt_case_inputs(n_rows, n_cols, nnz, n_dense_cols, value_dtype, index_dtype):
+    device = torch.device("cuda")
+    data, indices, indptr = _build_random_csr(n_rows, n_cols, nnz, value_dtype, index_dtype, device)
+    B = _build_random_dense((n_cols, n_dense_cols), value_dtype, device)
+    shape = (n_rows, n_cols)
+    reference = _golden(data, indices, indptr, shape, 0, B).to(value_dtype)
+    return data, indices, indptr, B, shape, reference
+
+
+def _first_call_ms(fn):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1000.0
+
+
+def _rocsparse_ms(data, indices, indptr, shape, B, warmup, iters):
+    try:
+        a = _vendor_operand(data, indices, indptr, shape, 0)
+        return _benchmark_cuda_op(lambda: torch.sparse.mm(a, B), warmup, iters)[1]
+    except RuntimeError:
+        return None
+
+
+def benchmark_spmm_opt_case(n_rows=4096, n_cols=4096, nnz=65536, n_dense_cols=32, value_dtype=torch.float32,
+                            index_dtype=torch.int32, warmup=20, iters=200, run_cusparse=True):
+    """SpMM base vs opt against the float64 reference (spmm_csr.py:4313-4437):
+    base is flagsparse_spmm_csr, opt flagsparse_spmm_csr_opt over a prepared
+    object, pt_ms torch.sparse.mm (rocSPARSE); cu_ms stays None (no
+    CuPy/cuSPARSE on ROCm)."""
+    from .spmm_csr import flagsparse_spmm_csr, flagsparse_spmm_csr_opt, prepare_spmm_csr_opt  # noqa: PLC0415
+
+    if value_dtype not in (torch.float32, torch.float64):
+        raise TypeError("benchmark_spmm_opt_case only supports float32 and float64")
+    data, indices, indptr, B, shape, ref = _opt_case_inputs(n_rows, n_cols, nnz, n_dense_cols, value_dtype,
+                                                            index_dtype)
+    prepared = prepare_spmm_csr_opt(data, indices, indptr, shape)
+    base_first_call_ms = _first_call_ms(lambda: flagsparse_spmm_csr(data, indices, indptr, B, shape))
+    base_values, base_ms = _benchmark_cuda_op(lambda: flagsparse_spmm_csr(data, indices, indptr, B, shape),
+                                              warmup, iters)
+    opt_first_call_ms = _first_call_ms(lambda: flagsparse_spmm_csr_opt(B=B, prepared=prepared))
+    opt_values, opt_ms = _benchmark_cuda_op(lambda: flagsparse_spmm_csr_opt(B=B, prepared=prepared), warmup, iters)
+    pt_ms = _rocsparse_ms(data, indices, indptr, shape, B, warmup, iters)
+    err_base = _opt_reference_error(base_values, ref, value_dtype)
+    err_opt = _opt_reference_error(opt_values, ref, value_dtype)
+    return {
+        "parameters": {"n_rows": n_rows, "n_cols": n_cols, "nnz": nnz, "n_dense_cols": n_dense_cols,
+                       "value_dtype": str(value_dtype), "index_dtype": str(index_dtype)},
+        "performance": {
+            "base_ms": base_ms, "base_first_call_ms": base_first_call_ms, "opt_ms": opt_ms,
+            "opt_first_call_ms": opt_first_call_ms, "pt_ms": pt_ms, "cu_ms": None,
+            "opt_vs_base": _speedup(base_ms, opt_ms), "opt_vs_pt": _speedup(pt_ms, opt_ms), "opt_vs_cu": None,
+        },
+        "verification": {"err_base": err_base, "err_opt": err_opt, "base_ok": err_base <= 1.0,
+                         "opt_ok": err_opt <= 1.0, "status": "PASS" if err_opt <= 1.0 else "FAIL"},
+        "backend_status": {"long_row_fallback_used": bool(prepared.long_row_fallback_only),
+                           "flagsparse_internal_route": "hip-row-bins",
+                           "cusparse_unavailable_reason": _VENDOR_NOTE if run_cusparse else None},
+        "samples": {"base": base_values, "opt": opt_values, "reference": ref},
+    }
+
+
+def benchmark_spmm_opt_alg2_case(n_rows=4096, n_cols=4096, nnz=65536, n_dense_cols=32, value_dtype=torch.float32,
+                                 index_dtype=torch.int32, warmup=20, iters=200, run_cusparse=True):
+    """SpMM base, opt and opt-alg2 against the float64 reference
+    (spmm_csr_opt_alg2.py:1252-1400); torch_ms is torch.sparse.mm
+    (rocSPARSE), the cusparse entries stay None with the reason set."""
+    from .spmm_csr import (  # noqa: PLC0415
+        flagsparse_spmm_csr,
+        flagsparse_spmm_csr_opt,
+        flagsparse_spmm_csr_opt_alg2,
+        prepare_spmm_csr_opt,
+        prepare_spmm_csr_opt_alg2,
+    )
+
+    if value_dtype not in (torch.float32, torch.float64):
+        raise TypeError("benchmark_spmm_opt_alg2_case only supports float32 and float64")
+    data, indices, indptr, B, shape, ref = _opt_case_inputs(n_rows, n_cols, nnz, n_dense_cols, value_dtype,
+                                                            index_dtype)
+    prepared_opt = prepare_spmm_csr_opt(data, indices, indptr, shape)
+    prepared_alg2 = prepare_spmm_csr_opt_alg2(data, indices, indptr, shape)
+    base_values, base_ms = _benchmark_cuda_op(lambda: flagsparse_spmm_csr(data, indices, indptr, B, shape),
+                                              warmup, iters)
+    opt_values, opt_ms = _benchmark_cuda_op(lambda: flagsparse_spmm_csr_opt(B=B, prepared=prepared_opt), warmup,
+                                            iters)
+    alg2_values, alg2_ms = _benchmark_cuda_op(lambda: flagsparse_spmm_csr_opt_alg2(B=B, prepared=prepared_alg2),
+                                              warmup, iters)
+    torch_ms = _rocsparse_ms(data, indices, indptr, shape, B, warmup, iters)
+    return {
+        "parameters": {"n_rows": n_rows, "n_cols": n_cols, "nnz": nnz, "n_dense_cols": n_dense_cols,
+                       "value_dtype": str(value_dtype), "index_dtype": str(index_dtype)},
+        "performance": {"base_ms": base_ms, "opt_ms": opt_ms, "opt_alg2_ms": alg2_ms, "torch_ms": torch_ms,
+                        "cusparse_ms": None},
+        "verification": {
+            "base_vs_torch_err": _opt_reference_error(base_values, ref, value_dtype),
+            "opt_vs_torch_err": _opt_reference_error(opt_values, ref, value_dtype),
+            "opt_alg2_vs_torch_err": _opt_reference_error(alg2_values, ref, value_dtype),
+            "base_vs_cusparse_err": None,
+            "opt_vs_cusparse_err": None,
+            "opt_alg2_vs_cusparse_err": None,
+        },
+        "backend_status": {"cusparse_unavailable_reason": _VENDOR_NOTE if run_cusparse else None},
+        "samples": {"base_triton": base_values, "opt_triton": opt_values, "opt_alg2_triton": alg2_values,
+                    "torch_ref": ref, "cusparse_ref": None},
+    }

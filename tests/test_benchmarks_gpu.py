@@ -63,3 +63,28 @@ def test_benchmark_empty_and_graph_timer(device):
     y = torch.empty_like(x)
     ms = benchmarks._benchmark_cuda_graph_op(lambda: torch.mul(x, 2.0, out=y), graph_batch=4, warmup=1, repeats=3)
     assert ms > 0 and torch.equal(y, x * 2.0)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
+def test_benchmark_spmm_opt_cases(device, dtype):
+    torch.manual_seed(3)
+    r = fs.benchmark_spmm_opt_case(n_rows=500, n_cols=400, nnz=6000, n_dense_cols=24, value_dtype=dtype, warmup=1,
+                                   iters=2)
+    assert SECTIONS <= set(r)
+    v, p = r["verification"], r["performance"]
+    assert v["base_ok"] and v["opt_ok"] and v["status"] == "PASS"
+    assert p["base_ms"] > 0 and p["opt_ms"] > 0 and p["cu_ms"] is None
+    assert r["samples"]["opt"].shape == (500, 24) and r["samples"]["reference"].dtype == dtype
+    assert torch.equal(r["samples"]["base"], r["samples"]["opt"])  # the same kernel and plan
+    torch.manual_seed(4)
+    r2 = fs.benchmark_spmm_opt_alg2_case(n_rows=300, n_cols=700, nnz=5000, n_dense_cols=40, value_dtype=dtype,
+                                         warmup=1, iters=2)
+    v2 = r2["verification"]
+    for key in ("base_vs_torch_err", "opt_vs_torch_err", "opt_alg2_vs_torch_err"):
+        assert v2[key] <= 1.0, key
+    assert v2["opt_alg2_vs_cusparse_err"] is None and r2["backend_status"]["cusparse_unavailable_reason"]
+    assert r2["samples"]["opt_alg2_triton"].shape == (300, 40) and r2["performance"]["opt_alg2_ms"] > 0
+    with pytest.raises(TypeError, match="only supports float32 and float64"):
+        fs.benchmark_spmm_opt_case(value_dtype=torch.bfloat16)
+    with pytest.raises(TypeError, match="only supports float32 and float64"):
+        fs.benchmark_spmm_opt_alg2_case(value_dtype=torch.float16)
