@@ -74,6 +74,17 @@ from .spmm_csr import (
     prepare_spmm_csr_route,
     resolve_spmm_csr_algorithm,
 )
+from .spmm_coo import (
+    SPMM_COO_ALGORITHMS,
+    PreparedCooSpmmRoute,
+    SpmmCooAlgorithm,
+    SpmmCooAlgorithmUnavailable,
+    flagsparse_spmm_coo,
+    flagsparse_spmm_coo_run,
+    list_spmm_coo_algorithms,
+    prepare_spmm_coo_route,
+    resolve_spmm_coo_algorithm,
+)
 from .mtx import load_mtx_to_csr_torch
 from .spmv_bsr import PreparedBsrSpmv, flagsparse_spmv_bsr, prepare_spmv_bsr
 from .spmv_coo import (
@@ -90,6 +101,15 @@ from .spmv_csc import PreparedCscSpmv, flagsparse_spmv_csc, prepare_spmv_csc
 __version__ = "0.1.0"
 
 __all__ = [
+    "SPMM_COO_ALGORITHMS",
+    "PreparedCooSpmmRoute",
+    "SpmmCooAlgorithm",
+    "SpmmCooAlgorithmUnavailable",
+    "flagsparse_spmm_coo",
+    "flagsparse_spmm_coo_run",
+    "list_spmm_coo_algorithms",
+    "prepare_spmm_coo_route",
+    "resolve_spmm_coo_algorithm",
     "benchmark_spmm_opt_case",
     "benchmark_spmm_opt_alg2_case",
     "PreparedCoo",
