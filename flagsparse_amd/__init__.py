@@ -47,6 +47,7 @@ from .benchmarks import (
     benchmark_spmm_opt_alg2_case,
     benchmark_spmm_opt_case,
     benchmark_spmv_case,
+    comprehensive_spmm_test,
 )
 from .autograd import CsrMatrix, flagsparse_spmm_csr_autograd, spmm_csr_autograd
 from .bsr import csr_to_bsr, flagsparse_spmm_bsr, quantize_bsr_fp8
@@ -101,6 +102,7 @@ from .spmv_csc import PreparedCscSpmv, flagsparse_spmv_csc, prepare_spmv_csc
 __version__ = "0.1.0"
 
 __all__ = [
+    "comprehensive_spmm_test",
     "SPMM_COO_ALGORITHMS",
     "PreparedCooSpmmRoute",
     "SpmmCooAlgorithm",

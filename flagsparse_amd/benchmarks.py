@@ -419,3 +419,13 @@ def benchmark_spmm_opt_alg2_case(n_rows=4096, n_cols=4096, nnz=65536, n_dense_co
         "samples": {"base_triton": base_values, "opt_triton": opt_values, "opt_alg2_triton": alg2_values,
                     "torch_ref": ref, "cusparse_ref": None},
     }
+
+
+def comprehensive_spmm_test(n_rows=4096, n_cols=4096, nnz=65536, n_dense_cols=32, dtype=torch.float32,
+                            index_dtype=torch.int32, warmup=20, iters=200, block_n=None, block_nnz=None,
+                            max_segments=None, run_cusparse=True, op="non"):
+    """Full SpMM benchmark entry for one configuration: benchmark_spmm_case
+    with the value dtype passed as ``dtype`` (spmm_csr.py:4796-4825)."""
+    return benchmark_spmm_case(n_rows=n_rows, n_cols=n_cols, nnz=nnz, n_dense_cols=n_dense_cols, value_dtype=dtype,
+                               index_dtype=index_dtype, warmup=warmup, iters=iters, block_n=block_n,
+                               block_nnz=block_nnz, max_segments=max_segments, run_cusparse=run_cusparse, op=op)

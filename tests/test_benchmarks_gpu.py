@@ -88,3 +88,12 @@ def test_benchmark_spmm_opt_cases(device, dtype):
         fs.benchmark_spmm_opt_case(value_dtype=torch.bfloat16)
     with pytest.raises(TypeError, match="only supports float32 and float64"):
         fs.benchmark_spmm_opt_alg2_case(value_dtype=torch.float16)
+
+
+def test_comprehensive_spmm_test_is_the_spmm_case(device):
+    torch.manual_seed(5)
+    r = fs.comprehensive_spmm_test(n_rows=200, n_cols=150, nnz=1500, n_dense_cols=16, dtype=torch.float64, warmup=1,
+                                   iters=2, op="trans")
+    assert SECTIONS <= set(r) and r["parameters"]["op"] == "trans"
+    assert r["parameters"]["value_dtype"] == str(torch.float64)
+    assert r["verification"]["triton_max_relative_error"] <= REL_TOL[torch.float64]

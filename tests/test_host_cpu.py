@@ -156,3 +156,50 @@ def test_power_law_graph_is_symmetric_and_normalised():
     deg = np.diff(ip)
     r = np.repeat(np.arange(2000), deg)
     np.testing.assert_allclose(v, 1 / np.sqrt(deg[r] * deg[ix]))
+
+
+# The reference's exported names on the CSR SpMV / SpMM / SDDMM path and its
+# adjacent formats (src/flagsparse/__init__.py __all__), each of which this
+# package provides under the same name.
+REFERENCE_PATH_NAMES = (
+    "flagsparse_spmv_csr", "prepare_spmv_csr", "PreparedCsrSpmv",
+    "flagsparse_spmv_csc", "prepare_spmv_csc", "PreparedCscSpmv",
+    "flagsparse_spmv_coo", "prepare_spmv_coo", "PreparedCoo",
+    "flagsparse_spmv_coo_tocsr", "prepare_spmv_coo_tocsr", "coo_to_csr_for_spmv",
+    "flagsparse_spmv_bsr", "prepare_spmv_bsr", "PreparedBsrSpmv",
+    "flagsparse_spmm_csr", "flagsparse_spmm_csr_opt", "prepare_spmm_csr_opt", "PreparedCsrSpmmOpt",
+    "flagsparse_spmm_csr_opt_alg1", "prepare_spmm_csr_opt_alg1", "flagsparse_spmm_csr_opt_alg1_preprocess",
+    "prepare_spmm_csr_opt_alg1_preprocess", "flagsparse_spmm_csr_opt_alg2", "prepare_spmm_csr_opt_alg2",
+    "flagsparse_spmm_csr_opt_alg2_preprocess", "prepare_spmm_csr_opt_alg2_preprocess", "PreparedCsrSpmmOptAlg2",
+    "SPMM_CSR_ALGORITHMS", "SpmmCsrAlgorithm", "SpmmCsrAlgorithmUnavailable", "PreparedCsrSpmmRoute",
+    "prepare_spmm_csr_route", "flagsparse_spmm_csr_run", "list_spmm_csr_algorithms", "resolve_spmm_csr_algorithm",
+    "flagsparse_spmm_coo", "flagsparse_spmm_coo_run", "prepare_spmm_coo_route", "PreparedCooSpmmRoute",
+    "SPMM_COO_ALGORITHMS", "SpmmCooAlgorithm", "SpmmCooAlgorithmUnavailable", "list_spmm_coo_algorithms",
+    "resolve_spmm_coo_algorithm",
+    "flagsparse_alpha_spmm_alg1", "flagsparse_alpha_spmm_alg1_tle", "flagsparse_alpha_spmm_alg1_tle_opt",
+    "flagsparse_alpha_spmm_alg1_tle_opt2", "prepare_alpha_spmm_alg1", "prepare_alpha_spmm_alg1_tle",
+    "prepare_alpha_spmm_alg1_tle_opt", "prepare_alpha_spmm_alg1_tle_opt2", "PreparedAlphaSpmmAlg1",
+    "build_alpha_spmm_alg1_tle_opt_meta", "build_alpha_spmm_alg1_tle_opt2_meta",
+    "is_alpha_spmm_alg1_tle_available", "is_alpha_spmm_alg1_tle_opt_available",
+    "is_alpha_spmm_alg1_tle_opt2_available", "alpha_spmm_alg1_tle_unavailable_reason",
+    "alpha_spmm_alg1_tle_opt_unavailable_reason", "alpha_spmm_alg1_tle_opt2_unavailable_reason",
+    "flagsparse_sddmm_csr", "prepare_sddmm_csr", "SDDMMPrepared",
+    "benchmark_spmv_case", "benchmark_spmm_case", "benchmark_sddmm_case", "benchmark_spmm_opt_case",
+    "benchmark_spmm_opt_alg2_case", "comprehensive_spmm_test",
+    "SUPPORTED_VALUE_DTYPES", "SUPPORTED_INDEX_DTYPES", "error_metrics", "validation_metrics",
+    "load_mtx_to_csr_torch",
+)
+
+
+def test_reference_path_names_are_exported():
+    missing = [n for n in REFERENCE_PATH_NAMES if n not in fs.__all__ or not hasattr(fs, n)]
+    assert not missing, missing
+    assert len(set(fs.__all__)) == len(fs.__all__)  # no duplicates in __all__
+
+
+def test_availability_queries_and_registries_need_no_device():
+    assert fs.is_alpha_spmm_alg1_tle_opt2_available() and fs.alpha_spmm_alg1_tle_opt_unavailable_reason() == ""
+    assert fs.list_spmm_coo_algorithms(op="trans") == ("coo_rowrun", "coo_atomic", "spmm_coo_alg1")
+    assert fs.resolve_spmm_coo_algorithm("rowrun", 0, torch.float64).display_name == "COORowRun"
+    with pytest.raises(ValueError, match="op must be one of"):
+        fs.list_spmm_coo_algorithms(op="sideways")
