@@ -169,7 +169,11 @@ int fsp_spmv_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
  * gathers from global memory) and runs the rows with `lanes` (4, 8 or 16)
  * lanes per row, reading indices and data as 4-element chunks (both aligned
  * to 4 elements).  Parts must not share rows; rows outside every part are not
- * written.  Accumulation as fsp_spmv_csr; nnz = length of indices / data. */
+ * written.  Accumulation as fsp_spmv_csr; nnz = length of indices / data.
+ * The table is read on the device and not checked: every row_end must be a
+ * valid row of indptr_dev (indptr has row_end + 1 entries), col_lo <= col_hi
+ * must lie within x, and the rows' column ids within their part's window.
+ * The Python layer builds it from a validated batch (CsrBatch.parts). */
 int fsp_spmv_csr_parts(int32_t value_type, int32_t index_type, int32_t indptr_type,
                        const void* indptr_dev, const void* indices_dev, const void* data_dev,
                        const void* x_dev, void* y_dev, int64_t nnz, int64_t n_parts,
