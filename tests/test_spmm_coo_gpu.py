@@ -147,3 +147,19 @@ def test_out_meta_and_empty(device):
     ei = torch.empty(0, dtype=torch.int32, device=device)
     Z = fs.flagsparse_spmm_coo(e, ei, ei, torch.randn(6, 3, device=device), (4, 6))
     assert Z.shape == (4, 3) and torch.all(Z == 0)
+
+
+def test_float32_is_computed_in_float64(device):
+    """The reference computes float32 COO SpMM in float64 and rounds once
+    (spmm_coo.py:146-153): every element is then within one float32 ulp of
+    the float64 product rounded to float32 (float32 accumulation over these
+    ~40-term rows would not be)."""
+    M, K, N = 400, 300, 32
+    row, col, v = _coo(M, K, 16000, seed=21)
+    data, r, c = _dev(row, col, v, torch.float32, torch.int32, device)
+    b = np.random.default_rng(22).standard_normal((K, N))
+    C = fs.flagsparse_spmm_coo(data, r, c, torch.tensor(b, dtype=torch.float32, device=device), (M, K))
+    ref32 = _expected(row, col, v, b, M, K, "non", torch.float32).astype(np.float32)
+    got = C.cpu().numpy()
+    ulp = np.spacing(np.abs(ref32)).astype(np.float64)
+    assert np.all(np.abs(got.astype(np.float64) - ref32.astype(np.float64)) <= ulp)
