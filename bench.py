@@ -111,6 +111,26 @@ def bench_spmm_power_law(dev, steps, warmup):
     }
 
 
+def bench_gather_probe(dev, n_rows, row_bytes, n_gathers, steps, warmup):
+    """Random whole-row reads from a table of B's footprint (fsp_gather_rows_probe),
+    one per nonzero: the practical ceiling of the SpMM's B-row gathers, timed
+    like the SpMM (HIP events on the launch stream).  Returns (ms, GB/s)."""
+    from flagsparse_amd import _lib  # noqa: PLC0415
+
+    table = torch.empty(n_rows * row_bytes // 4, dtype=torch.float32, device=dev).normal_()
+    sink = torch.zeros(1, dtype=torch.float32, device=dev)
+    lib = _lib.load()
+    stream = torch.cuda.current_stream(dev)
+
+    def run():
+        _lib.check(lib.fsp_gather_rows_probe(table.data_ptr(), n_rows, row_bytes, n_gathers, sink.data_ptr(),
+                                             stream.cuda_stream), "fsp_gather_rows_probe")
+
+    _, ev = time_steps(run, steps, warmup, stream)
+    ms = statistics.median(ev)
+    return ms, n_gathers * row_bytes / (ms * 1e-3) / 1e9
+
+
 def graph_time_ms(fn, dev, per_graph=64, reps=15):
     """Median device time per call of `fn` replayed from a captured graph of
     `per_graph` calls: the kernels back to back, without host launch gaps."""
@@ -570,6 +590,17 @@ def main():
             "gbs": round(r["trans_bytes"] / (r["trans_ms"] * 1e-3) / 1e9, 2),
             "ms_per_step": round(r["trans_ms"], 5),
         },
+    }
+    # the gather ceiling: the SpMM's B-row reads (nnz x 64 x 4 B) against the
+    # same number of random 256-byte row reads from a 64 MiB table
+    probe_ms, probe_gbs = bench_gather_probe(dev, 262144, 256, r["nnz"], args.steps, args.warmup)
+    spmm_gather_gbs = r["nnz"] * 256 / (kern_ms * 1e-3) / 1e9
+    line["roofline"]["gather_probe"] = {
+        "what": "random 256-byte row reads from a 64 MiB table, one per nonzero (fsp_gather_rows_probe)",
+        "ms": round(probe_ms, 5),
+        "gbs": round(probe_gbs, 1),
+        "spmm_gather_gbs": round(spmm_gather_gbs, 1),
+        "frac": round(spmm_gather_gbs / probe_gbs, 4),
     }
     extras = set(e for e in args.only.split(",") if e) if args.only else {
         "copy", "spmv_large", "spmv_small", "spmm_tuned", "autograd", "bsr", "batch", "vendor"}

@@ -72,3 +72,12 @@ def test_only_torch_hip_runtime_is_mapped():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(_lib.FlagSparseLibraryError, match="missing"):
         _lib.load(str(tmp_path / "nope.so"))
+
+
+def test_gather_probe_rejects_arguments_without_a_device():
+    lib = _lib.load()
+    assert lib.fsp_gather_rows_probe(None, 64, 256, 10, None, None) == -1  # NULL table
+    assert lib.fsp_gather_rows_probe(None, 64, 24, 10, None, None) == -1
+    assert b"row_bytes" in lib.fsp_last_error()
+    assert lib.fsp_gather_rows_probe(None, 1 << 32, 256, 10, None, None) == -1
+    assert b"n_rows" in lib.fsp_last_error()
