@@ -118,6 +118,9 @@ class CsrBatch:
         """Pack a list of (data, indices, indptr, shape) CSR matrices (copies)."""
         if not matrices:
             raise ValueError("a batch needs at least one matrix")
+        dtypes = {m[0].dtype for m in matrices}
+        if len(dtypes) != 1:
+            raise TypeError(f"every matrix of a batch needs the same value dtype, got {sorted(map(str, dtypes))}")
         shapes = [(int(s[0]), int(s[1])) for _, _, _, s in matrices]
         rows = np.concatenate([[0], np.cumsum([s[0] for s in shapes])])
         cols = np.concatenate([[0], np.cumsum([s[1] for s in shapes])])

@@ -185,6 +185,11 @@ def test_batch_validation(device):
     assert torch.equal(good.spmv(x + 1, data=strided), good.spmv(x + 1))
     with pytest.raises(ValueError, match="output tensor"):
         good.graph_spmv(x, None)
+    # one value dtype per batch: no silent promotion when packing
+    m0 = (data[:3], indices[:3], torch.tensor([0, 3], dtype=indptr.dtype, device=device), (1, 8))
+    m1 = (data[:2].double(), indices[:2], torch.tensor([0, 2], dtype=indptr.dtype, device=device), (1, 8))
+    with pytest.raises(TypeError, match="same value dtype"):
+        CsrBatch.from_matrices([m0, m1])
 
 
 def test_autotune_picks_and_caches(device, tmp_path, monkeypatch):
