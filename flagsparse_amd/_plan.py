@@ -23,7 +23,7 @@ The reference rebuilds its buckets with ``torch.nonzero`` on every call
 """
 
 import ctypes
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 import torch
@@ -162,22 +162,21 @@ class CsrPlan:
     long_seg_ptr: torch.Tensor
     n_segments: int
     n_long_rows: int
-    _partials: dict = field(default_factory=dict)
 
     @property
     def bin_sizes(self):
         return list(self.bin_counts) + [self.n_long_rows]
 
     def partials(self, n_dense, acc_dtype, device):
-        """Scratch of n_segments x n_dense partial sums (cached per shape)."""
+        """Scratch of n_segments x n_dense partial sums for ONE launch.
+
+        A fresh buffer from torch's stream-ordered caching allocator on the
+        current stream (the stream the launch goes to), so calls that share a
+        plan on different streams never share partials; under graph capture it
+        comes from the graph's private pool."""
         if self.n_segments == 0:
             return None
-        key = (int(n_dense), acc_dtype)
-        buf = self._partials.get(key)
-        if buf is None:
-            buf = torch.empty(self.n_segments * max(1, int(n_dense)), dtype=acc_dtype, device=device)
-            self._partials[key] = buf
-        return buf
+        return torch.empty(self.n_segments * max(1, int(n_dense)), dtype=acc_dtype, device=device)
 
     def struct(self, partials=None, max_blocks=0, xcd_groups=0, variant=0):
         """The C plan.  max_blocks > 0 makes the launch persistent: at most

@@ -16,48 +16,32 @@ from . import _plan
 from ._matrix import CsrHandle
 
 SUPPORTED_SDDMM_VALUE_DTYPES = (torch.float32, torch.float64)
-SUPPORTED_SDDMM_DIAGNOSTIC_VARIANTS = ("baseline", "acc64", "acc64_out64", "altreduce")
+# waves per workgroup of the SDDMM kernel (256-thread workgroups)
+_SDDMM_WAVES = 4
 
 
 class SDDMMPrepared:
-    """Prepared CSR pattern metadata for SDDMM (reference: sddmm_csr.py:23-47).
-    ``row_ids`` is kept for interface parity; the HIP kernel does not need it
-    (rows come from the row-bin plan)."""
+    """Prepared CSR pattern for SDDMM (reference type: sddmm_csr.py:23-47):
+    the validated pattern and its handle, which caches the row-bin plans."""
 
-    __slots__ = (
-        "indices",
-        "indptr",
-        "shape",
-        "n_rows",
-        "n_cols",
-        "nnz",
-        "row_ids",
-        "block_k",
-        "num_warps",
-        "_handle",
-    )
+    __slots__ = ("indices", "indptr", "shape", "n_rows", "n_cols", "nnz", "_handle")
 
-    def __init__(self, indices, indptr, shape, row_ids, block_k, num_warps):
+    def __init__(self, indices, indptr, shape):
         self.indices = indices
         self.indptr = indptr
         self.shape = (int(shape[0]), int(shape[1]))
         self.n_rows = self.shape[0]
         self.n_cols = self.shape[1]
         self.nnz = int(indices.numel())
-        self.row_ids = row_ids
-        self.block_k = int(block_k)
-        self.num_warps = int(num_warps)
         self._handle = CsrHandle(indices, indptr, self.shape)
 
 
-def _resolve_sddmm_launch_config(k):
-    """(lanes along K, waves per workgroup) of the HIP kernel for f32 data;
-    the reference returns (BLOCK_K, num_warps) (sddmm_csr.py:50-57)."""
-    lanes = 1
-    vk = -(-int(max(k, 1)) // 4)
-    while lanes < 64 and lanes < vk:
-        lanes *= 2
-    return lanes, 4
+def _sddmm_launch_meta(k, dtype):
+    """The meta keys the reference reports for its launch (block_k,
+    num_warps), filled with this kernel's: lanes along K and waves per
+    workgroup."""
+    elem = torch.empty(0, dtype=dtype).element_size()
+    return {"block_k": _plan.sddmm_lanes(max(int(k), 1), elem), "num_warps": _SDDMM_WAVES}
 
 
 def _prepare_sddmm_csr_pattern(indices, indptr, shape):
@@ -94,12 +78,10 @@ def _prepare_sddmm_csr_pattern(indices, indptr, shape):
 
 
 def prepare_sddmm_csr(indices, indptr, shape, k_hint=64):
-    """sddmm_csr.py:107-118."""
+    """sddmm_csr.py:107-118.  ``k_hint`` is part of the reference's signature;
+    the HIP launch picks its lanes from K at call time, so it is unused."""
     indices, indptr, shape = _prepare_sddmm_csr_pattern(indices, indptr, shape)
-    row_ids = None  # the row-grouped kernel takes rows from its row-bin plan
-    block_k, num_warps = _resolve_sddmm_launch_config(int(k_hint))
-    return SDDMMPrepared(indices=indices, indptr=indptr, shape=shape, row_ids=row_ids, block_k=block_k,
-                         num_warps=num_warps)
+    return SDDMMPrepared(indices=indices, indptr=indptr, shape=shape)
 
 
 def _validate_sddmm_dense_inputs(data, prepared, x, y):
@@ -150,10 +132,8 @@ def _run_sddmm_prepared(prepared, x, y, data, alpha, beta, out, allow_fallback=F
     a launch failure always raises."""
     out = _prepare_validated_sddmm_out(prepared, x, out)
     k_dim = int(x.shape[1])
-    lanes, waves = _resolve_sddmm_launch_config(k_dim)
     meta = {
-        "block_k": lanes,
-        "num_warps": waves,
+        **_sddmm_launch_meta(k_dim, x.dtype),
         "fallback_used": False,
         "variant": "baseline",
         "acc_dtype": "float64" if x.dtype == torch.float64 else "float32",
@@ -190,8 +170,7 @@ def flagsparse_sddmm_csr(
         if any(v is None for v in (indices, indptr, shape)):
             raise ValueError("indices, indptr, and shape are required when prepared is not provided")
         t_prepare0 = time.perf_counter()
-        k_hint = int(x.shape[1]) if (x is not None and x.ndim == 2) else 64
-        prepared = prepare_sddmm_csr(indices, indptr, shape, k_hint=k_hint)
+        prepared = prepare_sddmm_csr(indices, indptr, shape)
         prepare_ms = (time.perf_counter() - t_prepare0) * 1000.0
     elif not isinstance(prepared, SDDMMPrepared):
         raise TypeError("prepared must be a SDDMMPrepared instance")
@@ -206,8 +185,7 @@ def flagsparse_sddmm_csr(
             out.zero_()
         else:
             out.copy_(data * beta)
-        meta = {"prepare_ms": prepare_ms, "block_k": prepared.block_k, "num_warps": prepared.num_warps,
-                "fallback_used": False}
+        meta = {"prepare_ms": prepare_ms, **_sddmm_launch_meta(0, x.dtype), "fallback_used": False}
         if return_time and return_meta:
             return out, 0.0, meta
         if return_time:

@@ -69,8 +69,6 @@ class PreparedCsrSpmv:
         "index_fallback_policy",
         "index_fallback_applied",
         "index_fallback_reason",
-        "_baseline_compute_dtype",
-        "_baseline_data",
         "_plan",
     )
 
@@ -116,13 +114,6 @@ class PreparedCsrSpmv:
         self.index_fallback_policy = str(index_fallback_policy).lower()
         self.index_fallback_applied = bool(index_fallback_applied)
         self.index_fallback_reason = index_fallback_reason
-        if data.dtype in (torch.float16, torch.bfloat16):
-            self._baseline_compute_dtype = torch.float32
-        elif data.dtype == torch.float32:
-            self._baseline_compute_dtype = torch.float64
-        else:
-            self._baseline_compute_dtype = data.dtype
-        self._baseline_data = None
         self._plan = None
 
     def plan(self, config=None):
@@ -243,14 +234,8 @@ def prepare_spmv_csr(
     (data, kernel_indices, kernel_indptr, n_rows, n_cols, row_lengths, max_row_nnz), plan = _inspect_spmv_csr_matrix(
         data, indices, indptr, shape, index_fallback_policy=index_fallback_policy, plan_config=_plan.spmv_config()
     )
-    block_nnz_use = block_nnz
-    if max_segments is None:
-        max_segments_use = max((max_row_nnz + block_nnz_use - 1) // block_nnz_use, 1)
-        while max_segments_use > 2048 and block_nnz_use < 65536:
-            block_nnz_use *= 2
-            max_segments_use = max((max_row_nnz + block_nnz_use - 1) // block_nnz_use, 1)
-    else:
-        max_segments_use = max_segments
+    # block_nnz / max_segments keep the caller's launch hints in the prepared
+    # object's public slots; the HIP kernel takes its split from the plan
     prepared = PreparedCsrSpmv(
         data=data,
         kernel_indices=kernel_indices,
@@ -258,8 +243,8 @@ def prepare_spmv_csr(
         shape=shape,
         n_rows=n_rows,
         n_cols=n_cols,
-        block_nnz=block_nnz_use,
-        max_segments=max_segments_use,
+        block_nnz=block_nnz,
+        max_segments=max_segments,
         opt_max_segments=max_segments,
         row_lengths=row_lengths,
         max_row_nnz=max_row_nnz,
@@ -273,7 +258,8 @@ def prepare_spmv_csr(
 
 
 def _hip_spmv_csr_impl_prepared(prepared, x, out=None):
-    """Launch fsp_spmv_csr over the prepared plan; returns y (or `out`).
+    """Launch fsp_spmv_csr over the prepared plan; returns y (or `out`, a
+    contiguous validated result tensor the kernel writes directly).
     Replaces _triton_spmv_csr_impl_prepared (spmv_csr.py:702-746)."""
     dtype = prepared.data.dtype
     require_kernel_dtype(dtype)
@@ -360,21 +346,21 @@ def _spmv_prepared_with_int32_indices(prepared, reason):
     )
 
 
-def _run_spmv_prepared(prepared, x, use_opt=False, opt_buckets=None):
+def _run_spmv_prepared(prepared, x, use_opt=False, opt_buckets=None, out=None):
     # The HIP path is always the row-binned one; use_opt keeps its meaning of
     # "report the bucket count" (there is no separate unbucketed kernel).
-    return _hip_spmv_csr_impl_prepared(prepared, x)
+    return _hip_spmv_csr_impl_prepared(prepared, x, out=out)
 
 
-def _run_spmv_prepared_with_fallback(prepared, x, use_opt=False, opt_buckets=None):
+def _run_spmv_prepared_with_fallback(prepared, x, use_opt=False, opt_buckets=None, out=None):
     """spmv_csr.py:817-834."""
     try:
-        return _run_spmv_prepared(prepared, x, use_opt=use_opt, opt_buckets=opt_buckets)
+        return _run_spmv_prepared(prepared, x, use_opt=use_opt, opt_buckets=opt_buckets, out=out)
     except Exception as exc:
         if prepared.index_fallback_policy != "auto" or not _spmv_uses_int64_indices(prepared):
             raise
         fallback_prepared = _spmv_prepared_with_int32_indices(prepared, exc)
-        return _run_spmv_prepared(fallback_prepared, x, use_opt=use_opt)
+        return _run_spmv_prepared(fallback_prepared, x, use_opt=use_opt, out=out)
 
 
 def flagsparse_spmv_csr(
@@ -443,7 +429,9 @@ def flagsparse_spmv_csr(
     symbolic_ms = timer.mark()
     if symbolic_ms is not None and not (use_opt and prepared.supports_opt):
         symbolic_ms = 0.0
-    y = _run_spmv_prepared_with_fallback(prepared, x, use_opt=use_opt)
+    # a contiguous `out` is written by the kernel itself; a strided one gets a copy
+    direct = out if out is not None and out.is_contiguous() else None
+    y = _run_spmv_prepared_with_fallback(prepared, x, use_opt=use_opt, out=direct)
     compute_ms = timer.mark()
     op_total_ms = None if compute_ms is None else symbolic_ms + compute_ms
     if out is not None:
