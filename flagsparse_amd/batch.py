@@ -162,26 +162,35 @@ class CsrBatch:
         self.max_row = int(lengths.max().item()) if self.total_rows else 0
 
     # ------------------------------------------------------------- tuning
-    def params(self, kind, n_dense=1, dtype=None, tune=True):
-        """Kernel parameters of this batch for `kind` ("spmv" / "spmm"): the
-        cached winner of its shape class, else tuned now (tune=True) or the
-        defaults."""
-        dtype = dtype or self.data.dtype
+    def params(self, kind, n_dense=1, dtype=None, tune=True, data=None):
+        """Kernel parameters of this batch for `kind` ("spmv" / "spmm") run
+        with the values `data` (default: the batch's own): the cached winner
+        of its shape class, else tuned now (tune=True) or the defaults.  The
+        SpMV class includes whether the call's values sit on 4-element
+        boundaries (the vector-chunk and parts loops need it), so a winner
+        tuned on aligned values is never served to unaligned ones."""
+        data = self.data if data is None else data
+        dtype = dtype or data.dtype
+        if data.dtype != dtype:
+            data = data.to(dtype)
         cus = autotune.compute_units(self.device)
-        tag = autotune.spmv_class_tag(self.total_cols, self.indices, self.data) if kind == "spmv" else ""
+        tag = ""
+        if kind == "spmv":
+            aligned = _plan.chunk_aligned(self.indices, data)
+            tag = autotune.spmv_class_tag(self.total_cols, self.indices, data) + ("|a4" if aligned else "|u")
         key = autotune.shape_class(kind, dtype, self.indices.dtype, self.indptr.dtype, n_dense, self.total_rows,
                                    self.nnz, self.max_row, cus, extra=tag)
         p = self._params.get(key) or autotune.lookup(key)
         if p is None:
             if not tune:
                 return autotune.TuneParams()
-            p = self._tune(kind, key, n_dense, dtype)
+            p = self._tune(kind, key, n_dense, data)
         self._params[key] = p
         return p
 
-    def _tune(self, kind, key, n_dense, dtype):
-        elem = torch.empty(0, dtype=dtype).element_size()
-        data = self.data if self.data.dtype == dtype else self.data.to(dtype)
+    def _tune(self, kind, key, n_dense, data):
+        dtype = data.dtype
+        elem = data.element_size()
         chunk_ok = _plan.chunk_aligned(self.indices, data)
         if kind == "spmv":
             cands = autotune.batch_spmv_candidates(elem, self.total_cols, chunk_ok)
@@ -251,7 +260,11 @@ class CsrBatch:
         if x.ndim != 1 or x.dtype != data.dtype:
             raise ValueError("x must be 1D with the values' dtype")
         self._check_out(out, (self.total_rows,), data.dtype)
-        p = params or self.params("spmv", 1, data.dtype)
+        p = params or self.params("spmv", 1, data.dtype, data=data)
+        if p.variant in (4, 5) and not _plan.chunk_aligned(self.indices, data):
+            # the chunk loops read 4-element chunks: unaligned values take the
+            # pipelined loop (same row order, results within rounding)
+            p = autotune.TuneParams()
         if p.variant == 5:
             y = out if out is not None else torch.empty(self.total_rows, dtype=data.dtype, device=self.device)
             parts, _ = self.parts(p.part_nnz)
@@ -302,7 +315,7 @@ class CsrBatch:
 
     # ----------------------------------------------------- graph capture
     def _capture(self, fn):
-        fn()  # plans, partials and parameters exist before capture
+        fn()  # plans and parameters exist before capture
         torch.cuda.synchronize(self.device)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
@@ -346,7 +359,9 @@ class CsrBatch:
             raise ValueError("x must be 1D with the values' dtype")
         x = x.contiguous()
         self._check_out(out, (self.total_rows,), data.dtype)
-        p = params or self.params("spmv", 1, data.dtype)
+        p = params or self.params("spmv", 1, data.dtype, data=data)
+        if p.variant in (4, 5) and not _plan.chunk_aligned(self.indices, data):
+            p = autotune.TuneParams()  # as in spmv: no chunk loops on unaligned values
         y = out if out is not None else torch.empty(self.total_rows, dtype=data.dtype, device=self.device)
         if p.variant == 5:  # each matrix's own parts, one launch per matrix
             parts, start = self.parts(p.part_nnz)

@@ -213,6 +213,32 @@ def test_autotune_picks_and_caches(device, tmp_path, monkeypatch):
     assert torch.equal(b2.spmv(x), b.spmv_per_matrix(x, params=p))
 
 
+def test_unaligned_values_never_take_the_chunk_loops(device, tmp_path, monkeypatch):
+    """A parts-kernel (variant 5) winner of the aligned class is not served to
+    replacement values that start off a 4-element boundary."""
+    monkeypatch.setenv("FLAGSPARSE_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    autotune.clear_memory()
+    ip, ix, v, ro, no = S.csr_batch(n_mats=64, rows_lo=64, rows_hi=2048, seed=14, device=device)
+    b = CsrBatch(v, ix, ip, ro, None, no)
+    x = torch.rand(b.total_cols, device=device) * 2 - 1
+    b.params("spmv")
+    p5 = autotune.TuneParams(unroll=4, lanes_n=8, variant=5, part_nnz=1 << 13)
+    for key in list(b._params):
+        b._params[key] = p5  # the aligned class now holds a parts-kernel winner
+    ref = b.spmv(x)
+    buf = torch.empty(b.nnz + 1, dtype=b.data.dtype, device=device)
+    buf[1:] = b.data
+    shifted = buf[1:]
+    assert shifted.data_ptr() % 16 != 0
+    got = b.spmv(x, data=shifted)  # its own class: tuned without the chunk loops
+    assert b.params("spmv", data=shifted).variant not in (4, 5)
+    assert rel_err(got, ref.double().cpu().numpy()) <= 1e-6
+    # explicit chunk-loop parameters on unaligned values take the pipelined loop
+    assert rel_err(b.spmv(x, data=shifted, params=p5), ref.double().cpu().numpy()) <= 1e-6
+    assert rel_err(b.spmv_per_matrix(x, data=shifted, params=p5), ref.double().cpu().numpy()) <= 1e-6
+    autotune.clear_memory()
+
+
 # ------------------------------------- full batch: 4096 matrices, seed 4
 @pytest.fixture(scope="module")
 def full_batch():
