@@ -18,6 +18,8 @@
 // latency hides behind the gathers.  When every operand fits in 4 GiB
 // (OFF32), addresses are 32-bit byte offsets from the (uniform) base
 // pointers: one VGPR per address, which keeps occupancy at 8 waves/SIMD.
+#include <mutex>
+
 #include "fsp_device.hpp"
 #include "fsp_host.hpp"
 #include "fsp_plan.hpp"
@@ -301,14 +303,52 @@ static int launch_spmm_vec(spmm_args x, const fsp_csr_plan* plan, hipStream_t s)
 
 }  // namespace fsp
 
-using namespace fsp;
+// Scratch of the no-allocation SpMM entry: the long-row partials live in
+// this file-scope device array (one copy per GPU, held until the process
+// ends).  Host code never passes the array's name to a kernel: its device
+// address on each GPU comes from hipGetSymbolAddress, resolved once per device
+// with that device current and kept in scratch_dev.
+static __device__ __align__(256) unsigned char scratch[FSP_NOALLOC_SCRATCH_BYTES];
 
-extern "C" int fsp_spmm_csr(int32_t value_type, int32_t index_type, int32_t indptr_type, const void* indptr_dev,
-                            const void* indices_dev, const void* data_dev, int32_t perm_type, const void* perm_dev,
-                            const void* B_dev, int64_t ldb, void* C_dev, int64_t ldc, int64_t n_rows,
-                            int64_t n_dense, int64_t nnz, int64_t b_rows, int32_t lanes_n,
-                            const fsp_csr_plan* plan, void* stream) {
-  if (n_rows < 0 || n_dense < 0 || nnz < 0 || b_rows < 0) return fail(FSP_ERR_ARG, "fsp_spmm_csr: negative size");
+namespace fsp {
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kMaxScratchDevices];
+
+// Device address of `scratch` on the current device (resolved on first use).
+static int current_scratch(void* stream, void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(FSP_ERR_LAUNCH, "fsp_spmm_csr_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxScratchDevices) return fail(FSP_ERR_ARG, "fsp_spmm_csr_noalloc: device %d", dev);
+  if (stream != nullptr) {
+    int sdev = -1;
+    e = hipStreamGetDevice(as_stream(stream), &sdev);
+    if (e != hipSuccess) {
+      return fail(FSP_ERR_LAUNCH, "fsp_spmm_csr_noalloc: hipStreamGetDevice: %s", hipGetErrorString(e));
+    }
+    if (sdev != dev) {
+      return fail(FSP_ERR_ARG, "fsp_spmm_csr_noalloc: stream belongs to device %d, current device is %d", sdev, dev);
+    }
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      return fail(FSP_ERR_LAUNCH, "fsp_spmm_csr_noalloc: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    }
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return FSP_OK;
+}
+
+static int spmm_entry(int32_t value_type, int32_t index_type, int32_t indptr_type, const void* indptr_dev,
+                      const void* indices_dev, const void* data_dev, int32_t perm_type, const void* perm_dev,
+                      const void* B_dev, int64_t ldb, void* C_dev, int64_t ldc, int64_t n_rows, int64_t n_dense,
+                      int64_t nnz, int64_t b_rows, int32_t lanes_n, const fsp_csr_plan* plan, void* stream) {  if (n_rows < 0 || n_dense < 0 || nnz < 0 || b_rows < 0) return fail(FSP_ERR_ARG, "fsp_spmm_csr: negative size");
   int st = check_plan(plan, n_rows);
   if (st) return st;
   if (n_rows == 0 || n_dense == 0) return FSP_OK;
@@ -346,4 +386,46 @@ extern "C" int fsp_spmm_csr(int32_t value_type, int32_t index_type, int32_t indp
     });
   });
   return FSP_OK;
+}
+
+}  // namespace fsp
+
+using namespace fsp;
+
+extern "C" int fsp_spmm_csr(int32_t value_type, int32_t index_type, int32_t indptr_type, const void* indptr_dev,
+                            const void* indices_dev, const void* data_dev, int32_t perm_type, const void* perm_dev,
+                            const void* B_dev, int64_t ldb, void* C_dev, int64_t ldc, int64_t n_rows,
+                            int64_t n_dense, int64_t nnz, int64_t b_rows, int32_t lanes_n,
+                            const fsp_csr_plan* plan, void* stream) {
+  return spmm_entry(value_type, index_type, indptr_type, indptr_dev, indices_dev, data_dev, perm_type, perm_dev,
+                    B_dev, ldb, C_dev, ldc, n_rows, n_dense, nnz, b_rows, lanes_n, plan, stream);
+}
+
+extern "C" int fsp_spmm_csr_noalloc(int32_t value_type, int32_t index_type, int32_t indptr_type,
+                                    const void* indptr_dev, const void* indices_dev, const void* data_dev,
+                                    int32_t perm_type, const void* perm_dev, const void* B_dev, int64_t ldb,
+                                    void* C_dev, int64_t ldc, int64_t n_rows, int64_t n_dense, int64_t nnz,
+                                    int64_t b_rows, int32_t lanes_n, const fsp_csr_plan* plan, void* stream) {
+  if (plan == nullptr) return fail(FSP_ERR_ARG, "fsp_spmm_csr_noalloc: plan is NULL");
+  if (n_dense < 0 || plan->n_segments < 0) return fail(FSP_ERR_ARG, "fsp_spmm_csr_noalloc: negative size");
+  const int64_t acc_bytes = value_type == FSP_F64 ? 8 : 4;
+  // n_segments * n_dense * acc_bytes <= the scratch size, without overflow
+  if (plan->n_segments > 0 && n_dense > 0 &&
+      plan->n_segments > FSP_NOALLOC_SCRATCH_BYTES / acc_bytes / n_dense) {
+    return fail(FSP_ERR_WORKSPACE,
+                "fsp_spmm_csr_noalloc: %lld long-row segments x %lld dense columns x %lld bytes exceed the %lld-byte "
+                "static scratch; use fsp_spmm_csr with a caller-supplied partials buffer",
+                (long long)plan->n_segments, (long long)n_dense, (long long)acc_bytes,
+                (long long)FSP_NOALLOC_SCRATCH_BYTES);
+  }
+  fsp_csr_plan p = *plan;
+  p.partials_dev = nullptr;
+  p.partials_bytes = 0;
+  if (p.n_segments > 0 && n_rows > 0 && n_dense > 0 && nnz > 0) {
+    const int st = current_scratch(stream, &p.partials_dev);
+    if (st) return st;
+    p.partials_bytes = FSP_NOALLOC_SCRATCH_BYTES;
+  }
+  return spmm_entry(value_type, index_type, indptr_type, indptr_dev, indices_dev, data_dev, perm_type, perm_dev,
+                    B_dev, ldb, C_dev, ldc, n_rows, n_dense, nnz, b_rows, lanes_n, &p, stream);
 }

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSP_ABI_VERSION 10
+#define FSP_ABI_VERSION 11
 #define FSP_MAX_BINS 8
 
 typedef enum fsp_value_type {
@@ -195,6 +195,31 @@ int fsp_spmm_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
                  const void* B_dev, int64_t ldb, void* C_dev, int64_t ldc,
                  int64_t n_rows, int64_t n_dense, int64_t nnz, int64_t b_rows,
                  int32_t lanes_n, const fsp_csr_plan* plan, void* stream);
+
+/* Size of the static device scratch behind fsp_spmm_csr_noalloc (per GPU). */
+#define FSP_NOALLOC_SCRATCH_BYTES (16ll << 20)
+
+/* No counterpart in the reference: fsp_spmm_csr for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and result as
+ * fsp_spmm_csr, except that plan->partials_dev / partials_bytes are ignored:
+ * the long-row partials live in a file-scope device array of
+ * FSP_NOALLOC_SCRATCH_BYTES (16 MiB) that each GPU holds until the process
+ * ends.  Calls with plan->n_segments * n_dense * (8 for f64, else 4) above
+ * that size return FSP_ERR_WORKSPACE before anything is launched.  The call
+ * allocates nothing (no hipMalloc / hipFree, no allocating library routine);
+ * its first call on a device resolves the array's address there, and the
+ * stream must belong to the current device.  Every call on one GPU shares
+ * that GPU's array, which keeps the last call's partials (each call writes
+ * every partial it reads, so nothing needs clearing): calls that overlap on
+ * one GPU (other streams, other threads) race and must be serialised by the
+ * caller. */
+int fsp_spmm_csr_noalloc(int32_t value_type, int32_t index_type, int32_t indptr_type,
+                         const void* indptr_dev, const void* indices_dev,
+                         const void* data_dev, int32_t perm_type, const void* perm_dev,
+                         const void* B_dev, int64_t ldb, void* C_dev, int64_t ldc,
+                         int64_t n_rows, int64_t n_dense, int64_t nnz, int64_t b_rows,
+                         int32_t lanes_n, const fsp_csr_plan* plan, void* stream);
 
 /* ---- SDDMM ---------------------------------------------------------------
  * out[p] = alpha * <X[row(p), :], Y[col(p), :]> + beta * data_in[p]

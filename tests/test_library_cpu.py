@@ -81,3 +81,18 @@ def test_gather_probe_rejects_arguments_without_a_device():
     assert b"row_bytes" in lib.fsp_last_error()
     assert lib.fsp_gather_rows_probe(None, 1 << 32, 256, 10, None, None) == -1
     assert b"n_rows" in lib.fsp_last_error()
+
+
+def test_noalloc_spmm_rejects_oversized_partials_without_a_device():
+    lib = _lib.load()
+    st = _lib.FspCsrPlan()
+    st.n_bins = 0
+    st.unroll = 4
+    st.n_long_rows = 10
+    st.n_segments = _lib.NOALLOC_SCRATCH_BYTES // (64 * 4) + 1  # just past the scratch at N = 64, f32
+    status = lib.fsp_spmm_csr_noalloc(_lib.FSP_F32, _lib.FSP_I32, _lib.FSP_I32, None, None, None, 0, None, None, 64,
+                                      None, 64, 10, 64, 100, 10, 16, ctypes.byref(st), None)
+    assert status == -4
+    assert b"static scratch" in lib.fsp_last_error()
+    assert lib.fsp_spmm_csr_noalloc(_lib.FSP_F32, _lib.FSP_I32, _lib.FSP_I32, None, None, None, 0, None, None, 64,
+                                    None, 64, 10, 64, 100, 10, 16, None, None) == -1
