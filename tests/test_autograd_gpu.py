@@ -64,6 +64,32 @@ def test_bf16_forward_backward_vs_fp64(device):
     assert_rel(vals.grad, oracle.sddmm(ip, ix, dCn, Bn), torch.bfloat16, "bf16 dvalues")
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16], ids=["f32", "bf16"])
+def test_full_size_graph_forward_backward(device, dtype):
+    """BASELINE config 4 at its size: the 200000-node power-law graph (exponent
+    2.3, mean degree 14, D^-1/2 A D^-1/2 values, seed 3), B and dC of
+    200000 x 128.  float32: forward, dB = A^T dC and dvalues = SDDMM(dC, B)
+    within 1e-5 of the oracle on the float32-rounded inputs; bfloat16: within
+    1e-2 of the FP64 oracle on the unrounded inputs."""
+    n = 200000
+    ip, ix, v = _graph(n, seed=3)
+    rng = np.random.Generator(np.random.PCG64(3))
+    Bn, dCn = rng.standard_normal((n, 128)), rng.standard_normal((n, 128))
+    data, idx, ptr = device_csr(ip, ix, v, device, dtype)
+    mat = fs.CsrMatrix(idx, ptr, (n, n)).prepare_backward()
+    vals = data.clone().requires_grad_(True)
+    B = torch.tensor(Bn, dtype=dtype, device=device, requires_grad=True)
+    C = fs.spmm_csr_autograd(vals, B, mat)
+    C.backward(torch.tensor(dCn, dtype=dtype, device=device))
+    if dtype == torch.float32:
+        vr, Br, dCr = np_dtype_round(v, dtype), np_dtype_round(Bn, dtype), np_dtype_round(dCn, dtype)
+    else:
+        vr, Br, dCr = v, Bn, dCn
+    assert_rel(C, oracle.spmm(ip, ix, vr, Br, (n, n)), dtype, "fwd")
+    assert_rel(B.grad, oracle.spmm(ip, ix, vr, dCr, (n, n), "trans"), dtype, "dB")
+    assert_rel(vals.grad, oracle.sddmm(ip, ix, dCr, Br), dtype, "dvalues")
+
+
 def test_only_requested_gradients(device):
     ip, ix, v = _graph(300, seed=1)
     data, idx, ptr = device_csr(ip, ix, v, device, torch.float32)

@@ -273,6 +273,39 @@ def test_full_batch(device, full_batch):
     _oracle_sample(b, arrays, x, B, y, C)
 
 
+def test_full_batch_tuned_graph(device, full_batch, tmp_path, monkeypatch):
+    """BASELINE config 5 as specified: parameters from the autotuner, the
+    SpMV and the k = 32 SpMM each one launch captured in a HIP graph.  Every
+    replay equals the eager launch, which equals the per-matrix launch loop
+    bit for bit, and a sample of matrices meets the oracle."""
+    monkeypatch.setenv("FLAGSPARSE_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    b, arrays = full_batch
+    gen = torch.Generator(device=device).manual_seed(4)
+    x = torch.rand(b.total_cols, device=device, generator=gen) * 2 - 1
+    B = torch.randn(b.total_cols, 32, device=device, generator=gen)
+    pv = b.params("spmv")
+    pm = b.params("spmm", 32)
+    y = b.spmv(x)
+    C = b.spmm(B)
+    yg, Cg = torch.empty_like(y), torch.empty_like(C)
+    gv = b.graph_spmv(x, yg)
+    gm = b.graph_spmm(B, Cg)
+    for rep in range(3):
+        yg.fill_(float("nan"))
+        Cg.fill_(float("nan"))
+        gv.replay()
+        gm.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(yg, y), f"spmv replay {rep} (params {pv})"
+        assert torch.equal(Cg, C), f"spmm replay {rep} (params {pm})"
+        y2 = b.spmv(x)  # eager work between the replays
+        torch.cuda.synchronize()
+        assert torch.equal(y2, y)
+    assert torch.equal(b.spmv_per_matrix(x, params=pv), y)
+    assert torch.equal(b.spmm_per_matrix(B, params=pm), C)
+    _oracle_sample(b, arrays, x, B, y, C)
+
+
 def test_tune_spmv_single_matrix(device):
     from flagsparse_amd._matrix import CsrHandle
 
