@@ -243,20 +243,60 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void bsr_pack_b_fp8_kernel(const TI
   }
 }
 
-// absmax over a [rows, cols] (row stride ld) array, as float bits via an
-// integer max (non-negative floats order like their bit patterns).
+// |v| as float bits: non-negative floats order like their bit patterns, and
+// +Inf (0x7f800000) and NaN (above it) order above every finite value, so an
+// integer max over these bits is the absmax and carries a non-finite input
+// through (fmaxf would drop a NaN).
+template <typename TI>
+__device__ __forceinline__ unsigned int abs_bits(TI v) {
+  return __float_as_uint(static_cast<float>(v)) & 0x7fffffffu;
+}
+
+constexpr int kAbsmaxUnroll = 4;
+constexpr int kAbsmaxMaxBlocks = 1024;  // 4 workgroups per CU on 256 CUs; the loops stride the rest
+
+// absmax bits over a [rows, cols] (row stride ld) array: a bounded grid,
+// kAbsmaxUnroll independent loads in flight per thread, wave then workgroup
+// max in registers / LDS, ONE atomic per workgroup.
 template <typename TI>
 __global__ __launch_bounds__(FSP_WG_THREADS) void absmax_kernel(const TI* __restrict__ x, int64_t rows, int64_t cols,
                                                                 int64_t ld, unsigned int* __restrict__ out) {
-  float m = 0.f;
-  const int64_t total = rows * cols;
-  for (int64_t t = static_cast<int64_t>(blockIdx.x) * FSP_WG_THREADS + threadIdx.x; t < total;
-       t += static_cast<int64_t>(gridDim.x) * FSP_WG_THREADS) {
-    const int64_t r = t / cols;
-    m = fmaxf(m, fabsf(static_cast<float>(x[r * ld + (t - r * cols)])));
+  __shared__ unsigned int wave_max[FSP_WAVES];
+  unsigned int m = 0u;
+  if (ld == cols) {  // one flat range
+    const int64_t total = rows * cols;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * FSP_WG_THREADS;
+    int64_t t = static_cast<int64_t>(blockIdx.x) * FSP_WG_THREADS + threadIdx.x;
+    for (; t + (kAbsmaxUnroll - 1) * stride < total; t += kAbsmaxUnroll * stride) {
+      unsigned int a[kAbsmaxUnroll];
+#pragma unroll
+      for (int u = 0; u < kAbsmaxUnroll; ++u) a[u] = abs_bits(x[t + u * stride]);
+#pragma unroll
+      for (int u = 0; u < kAbsmaxUnroll; ++u) m = a[u] > m ? a[u] : m;
+    }
+    for (; t < total; t += stride) {
+      const unsigned int a = abs_bits(x[t]);
+      m = a > m ? a : m;
+    }
+  } else {  // strided rows: workgroups over rows, threads over columns
+    for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+      for (int64_t c = threadIdx.x; c < cols; c += FSP_WG_THREADS) {
+        const unsigned int a = abs_bits(x[r * ld + c]);
+        m = a > m ? a : m;
+      }
+    }
   }
-  for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+  for (int off = 32; off >= 1; off >>= 1) {
+    const unsigned int o = static_cast<unsigned int>(__shfl_xor(static_cast<int>(m), off));
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0) wave_max[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 1; w < FSP_WAVES; ++w) m = wave_max[w] > m ? wave_max[w] : m;
+    atomicMax(out, m);
+  }
 }
 
 __global__ void zero_u32_kernel(unsigned int* p) { p[0] = 0u; }
@@ -270,14 +310,20 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void bsr_quantize_fp8_kernel(const 
   for (int64_t b = (static_cast<int64_t>(blockIdx.x) * FSP_WG_THREADS + threadIdx.x) >> 6; b < nnzb;
        b += (static_cast<int64_t>(gridDim.x) * FSP_WG_THREADS) >> 6) {
     float v[4];
-    float m = 0.f;
+    unsigned int mb = 0u;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       v[i] = static_cast<float>(data[b * 256 + lane * 4 + i]);
-      m = fmaxf(m, fabsf(v[i]));
+      const unsigned int a = abs_bits(v[i]);
+      mb = a > mb ? a : mb;
     }
-    for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    const float scale = m > 0.f ? m / kE4m3Max : 1.f;
+    for (int off = 32; off >= 1; off >>= 1) {
+      const unsigned int o = static_cast<unsigned int>(__shfl_xor(static_cast<int>(mb), off));
+      mb = o > mb ? o : mb;
+    }
+    const float m = __uint_as_float(mb);
+    // a block holding Inf / NaN gets a NaN scale: its products come out NaN
+    const float scale = mb >= 0x7f800000u ? __builtin_nanf("") : (m > 0.f ? m / kE4m3Max : 1.f);
     uint32_t packed = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) packed |= static_cast<uint32_t>(e4m3_encode(v[i] / scale)) << (8 * i);
@@ -387,7 +433,12 @@ __global__ __launch_bounds__(64 * WPG) void bsr_spmm_kernel(
       cur = nxt;
     }
   }
-  const float bscale = KIND == 2 ? (b_absmax[0] > 0.f ? b_absmax[0] / kE4m3Max : 1.f) : 1.f;
+  float bscale = 1.f;
+  if constexpr (KIND == 2) {
+    // B holding Inf / NaN (absmax not finite) makes the whole product NaN
+    const float am = b_absmax[0];
+    bscale = __builtin_isfinite(am) ? (am > 0.f ? am / kE4m3Max : 1.f) : __builtin_nanf("");
+  }
   // D layout: lane l holds rows (l / 16) * 4 + i, column l % 16 of a tile.
 #pragma unroll
   for (int t = 0; t < kTilesPerWave; ++t) {
@@ -515,8 +566,10 @@ int fsp_absmax(int32_t value_type, const void* x_dev, int64_t rows, int64_t cols
   unsigned int* o = reinterpret_cast<unsigned int*>(out_dev);
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, o);
   if (rows * cols == 0) return check_launch("absmax_zero");
+  int64_t blocks = ld == cols ? ceil_div(rows * cols, int64_t(FSP_WG_THREADS) * kAbsmaxUnroll) : rows;
+  if (blocks > kAbsmaxMaxBlocks) blocks = kAbsmaxMaxBlocks;
   FSP_DISPATCH_VALUE(value_type, T, {
-    hipLaunchKernelGGL(absmax_kernel<T>, dim3(grid_1d(rows * cols)), dim3(FSP_WG_THREADS), 0, s,
+    hipLaunchKernelGGL(absmax_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(FSP_WG_THREADS), 0, s,
                        static_cast<const T*>(x_dev), rows, cols, ld, o);
   });
   return check_launch("absmax");

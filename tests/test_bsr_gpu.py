@@ -189,6 +189,53 @@ def test_bsr_spmm_fp8(device, case):
     assert err64 <= 0.1
 
 
+ABSMAX_SHAPES = [(1, 1), (3, 5), (1000, 1), (257, 129), (65536, 128), (4097, 1000)]
+
+
+@pytest.mark.parametrize("rows,cols", ABSMAX_SHAPES)
+@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16, torch.float16, torch.float64])
+@pytest.mark.parametrize("strided", [False, True])
+def test_absmax_exact(device, rows, cols, dtype, strided):
+    from flagsparse_amd import _lib
+
+    g = torch.Generator(device="cpu").manual_seed(rows * 7 + cols)
+    full = (torch.randn(rows, cols + (3 if strided else 0), generator=g) * 5).to(dtype).to(device)
+    x = full[:, :cols]
+    out = torch.full((1,), -1.0, dtype=torch.float32, device=device)
+    _lib.check(_lib.load().fsp_absmax(_lib.value_code(dtype), x.data_ptr(), rows, cols, int(full.stride(0)),
+                                      out.data_ptr(), _lib.stream_ptr(device)), "fsp_absmax")
+    assert float(out.item()) == float(x.abs().max().float().item())
+
+
+@pytest.mark.parametrize("bad", [float("inf"), float("-inf"), float("nan")])
+def test_non_finite_b_gives_nan_fp8_product(device, bad):
+    """A non-finite B makes the fp8 product NaN instead of silently zero
+    (absmax keeps Inf / NaN; the B scale is NaN)."""
+    n_rows, n_cols, nbc, bp, bc, bd, B = _spmm_case(SPMM_CASES[0], seed=5)
+    data, indices, indptr = _dev_bsr(bp, bc, bd, device, torch.float32)
+    codes, scales = FB.quantize_bsr_fp8(data)
+    Bt = S.dense(B, device).clone()
+    Bt[1, 2] = bad
+    _, absmax = FB.pack_dense_operand(Bt, torch.uint8)
+    got = float(absmax.item())
+    assert (np.isnan(got) if np.isnan(bad) else got == float("inf"))
+    C = FB.flagsparse_spmm_bsr(codes, indices, indptr, Bt, (n_rows, n_cols), scales=scales)
+    assert torch.isnan(C).all()
+    # a finite B through the same call is finite
+    C_ok = FB.flagsparse_spmm_bsr(codes, indices, indptr, S.dense(B, device), (n_rows, n_cols), scales=scales)
+    assert torch.isfinite(C_ok).all()
+
+
+def test_non_finite_block_gets_nan_scale(device):
+    n_rows, n_cols, nbc, bp, bc, bd, B = _spmm_case(SPMM_CASES[0], seed=6)
+    data, indices, indptr = _dev_bsr(bp, bc, bd, device, torch.float32)
+    data[1, 3, 4] = float("inf")
+    codes, scales = FB.quantize_bsr_fp8(data)
+    assert torch.isnan(scales[1]) and torch.isfinite(scales[0]) and torch.isfinite(scales[2:]).all()
+    C = FB.flagsparse_spmm_bsr(codes, indices, indptr, S.dense(B, device), (n_rows, n_cols), scales=scales)
+    assert not torch.isfinite(C).all()
+
+
 def test_bsr_spmm_validation(device):
     n_rows, n_cols, nbc, bp, bc, bd, B = _spmm_case(SPMM_CASES[0], seed=1)
     data, indices, indptr = _dev_bsr(bp, bc, bd, device, torch.bfloat16)
