@@ -66,34 +66,35 @@ def time_steps(fn, steps, warmup, stream):
 
 
 def bench_spmm_power_law(dev, steps, warmup):
+    """The headline step through the public prepared entry points: one
+    flagsparse_spmm_csr_opt(B=..., prepared=..., out=C) per step (non), and
+    one flagsparse_spmm_csr_run on the route prepared for op trans (the CSC
+    permutation is built by the first call, outside the timed steps)."""
     log("generating 262144^2 power-law CSR (seed 1)")
     ip, ix, v, B = synthetic.power_law_rows(seed=1)
     n = 262144
     nnz = int(ix.size)
     data, idx, ptr = synthetic.to_torch(ip, ix, v, dev, torch.float32, torch.int32)
     Bt = synthetic.dense(B, dev, torch.float32)
-    handle = CsrHandle(idx, ptr, (n, n))
+    prepared = fs.prepare_spmm_csr_opt(data, idx, ptr, (n, n))
     C = torch.empty((n, 64), dtype=torch.float32, device=dev)
-    lanes = _plan.lanes_for_dense(64, 4)
-    cfg = _plan.spmm_config(lanes)
-    plan = handle.plan(cfg)
     stream = torch.cuda.current_stream(dev)
-    log(f"nnz={nnz} bins={plan.bin_counts} long_rows={plan.n_long_rows} segments={plan.n_segments}")
 
     def step():
-        handle.spmm(data, Bt, out=C, lanes_n=lanes, config=cfg)
+        fs.flagsparse_spmm_csr_opt(B=Bt, prepared=prepared, out=C)
 
+    step()  # builds the row-bin plan (prepare-time work, outside the timed steps)
+    plan = prepared._handle.plan(_plan.spmm_config(_plan.lanes_for_dense(64, 4)))
+    log(f"nnz={nnz} bins={plan.bin_counts} long_rows={plan.n_long_rows} segments={plan.n_segments}")
     wall_ms, ev = time_steps(step, steps, warmup, stream)
     ms = wall_ms / steps
     nbytes = spmm_bytes(nnz, n, n, 64)
-    # trans on the cached CSC permutation (reported beside the headline)
-    Ct = torch.empty((n, 64), dtype=torch.float32, device=dev)
-    handle.csc()
-    handle.csc_plan(cfg)
+    route = fs.prepare_spmm_csr_route(data, idx, ptr, (n, n), op="trans")
 
     def step_t():
-        handle.spmm(data, Bt, transpose=True, out=Ct, lanes_n=lanes, config=cfg)
+        fs.flagsparse_spmm_csr_run(route, Bt)
 
+    step_t()  # builds the CSC permutation and its plan once
     wall_t, ev_t = time_steps(step_t, steps, warmup, stream)
     ms_t = wall_t / steps
     nbytes_t = nbytes + nnz * 4  # the permutation is read as well
@@ -375,8 +376,10 @@ def bench_vendor(dev, steps, warmup, inputs):
 def bench_bsr(dev, steps, warmup):
     """BSR 65536 x 65536 (16 x 16 blocks, 40 per block row, seed 2) times
     B 65536 x 128 on the matrix cores: bf16, f16 and fp8 e4m3 (per-block
-    absmax scales for A, one absmax scale for B).  Kernel time only (B
-    packed and A quantised beforehand)."""
+    absmax scales for A, one absmax scale for B).  event_ms_median is the
+    kernel alone (B packed and A quantised beforehand); per_call_ms is the
+    public flagsparse_spmm_bsr on the float32 B, which packs (fp8: takes the
+    absmax and quantises) B on every call."""
     from flagsparse_amd import _lib, bsr  # noqa: PLC0415
 
     log("generating BSR 65536^2, 40 blocks per block row (seed 2)")
@@ -418,10 +421,16 @@ def bench_bsr(dev, steps, warmup):
         if name == "fp8":
             nbytes += nnzb * 4
         tflops = flops / (med * 1e-3) / 1e12
+
+        def public_call():  # what a caller runs: B (f32) packed / quantised, then the kernel
+            fs.flagsparse_spmm_bsr(data, indices, indptr, Bt, (65536, 65536), scales=scales, out=C)
+
+        _, ev_call = time_steps(public_call, steps, warmup, stream)
         out[name] = {"event_ms_median": round(med, 5), "tflops": round(tflops, 2),
                      "mfma_util": round(tflops / peaks[name], 4), "bytes": nbytes,
                      "gbs": round(nbytes / (med * 1e-3) / 1e9, 2),
-                     "hbm_frac": round(nbytes / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                     "hbm_frac": round(nbytes / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "per_call_ms": round(statistics.median(ev_call), 5)}
         del data, Bp
     return out
 
@@ -470,46 +479,75 @@ def bench_batch(dev, steps, warmup):
     return out
 
 
-def cpu_baseline(inputs, budget_s=20.0):
-    """C oracle (one core) on the first rows of the same workload, scaled to
-    GB/s with the same algorithmic-byte model; plus torch.sparse on all cores."""
+def time_cpu(fn, warmup=3, min_reps=10, min_s=1.0, budget_s=30.0):
+    """Median wall time (s) of `fn` on the host: `warmup` untimed calls, then
+    at least `min_reps` timed ones, more while under `min_s` (short calls),
+    stopping once `budget_s` is spent (benchmark/performance_utils.py:81-93:
+    warm-ups, then the median; matrix construction outside)."""
+    for _ in range(warmup):
+        fn()
+    reps = []
+    t_start = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        fn()
+        reps.append(time.perf_counter() - t0)
+        spent = time.perf_counter() - t_start
+        if len(reps) >= min_reps and (spent >= min_s or len(reps) >= 10000):
+            break
+        if spent >= budget_s and len(reps) >= 3:
+            break
+    return statistics.median(reps), len(reps)
+
+
+def _cpu_info():
+    return {"os_cpu_count": os.cpu_count(), "torch_threads": torch.get_num_threads(), "cpu_model": _cpu_model(),
+            "torch_parallel_info": torch.__config__.parallel_info(), "torch_config": torch.__config__.show()}
+
+
+def cpu_baseline(inputs):
+    """The reference's CPU path on the box's host cores: torch.sparse.mm on
+    CPU tensors (its oracle call, ops/benchmarks.py:67-72 and
+    ops/spmm_csr.py:4520-4560) over the whole headline workload, and the
+    16384 x 16384 x 16 SpMV as torch.sparse.mm(A,
+    x.unsqueeze(1)); plus the C oracle (one core, long double) on the
+    headline workload as a side figure."""
     from oracle import oracle  # noqa: PLC0415
 
-    ip, ix, v, B = inputs
-    n = ip.size - 1
-    rows = 8192
-    res = {}
-    while True:
-        sub_ip = ip[: rows + 1]
-        sub_nnz = int(sub_ip[-1])
-        t0 = time.perf_counter()
-        oracle.spmm(sub_ip, ix[:sub_nnz], v[:sub_nnz], B, (rows, n))
-        dt = time.perf_counter() - t0
-        if dt > 2.0 or rows >= n:
-            break
-        rows = min(n, rows * 4)
-    # the sample's share of the workload's algorithmic bytes (by nonzeros)
-    nbytes = spmm_bytes(int(ix.size), n, n, 64) * (sub_nnz / ix.size)
-    res["oracle"] = {"value": nbytes / dt / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
-                     "sample": f"C oracle (long double) SpMM over the first {rows} rows ({sub_nnz} nnz) of the "
-                               f"262144^2 power-law matrix, N=64, {dt:.2f} s"}
     threads = os.cpu_count() or 1
     torch.set_num_threads(threads)
+    ip, ix, v, B = inputs
+    n = ip.size - 1
     A = torch.sparse_csr_tensor(torch.from_numpy(ip), torch.from_numpy(ix), torch.from_numpy(v.astype(np.float32)),
                                 size=(n, n))
     Bc = torch.from_numpy(B.astype(np.float32))
-    for _ in range(2):
-        torch.sparse.mm(A, Bc)
-    reps = []
-    t_start = time.perf_counter()
-    while len(reps) < 10 and time.perf_counter() - t_start < budget_s:
-        t0 = time.perf_counter()
-        torch.sparse.mm(A, Bc)
-        reps.append(time.perf_counter() - t0)
-    med = statistics.median(reps)
-    res["torch_sparse"] = {"value": spmm_bytes(int(ix.size), n, n, 64) / med / 1e9, "unit": "GB/s",
-                           "cores": torch.get_num_threads(), "ms": med * 1e3, "reps": len(reps),
-                           "cpu": _cpu_model()}
+    med, reps = time_cpu(lambda: torch.sparse.mm(A, Bc))
+    nbytes = spmm_bytes(int(ix.size), n, n, 64)
+    info = _cpu_info()
+    res = {"torch_sparse": {
+        "value": round(nbytes / med / 1e9, 3), "unit": "GB/s", "cores": torch.get_num_threads(), "kind": "reference",
+        "sample": f"torch.sparse.mm(A, B) on CPU tensors (the reference's CPU oracle path) over the whole headline "
+                  f"workload: 262144^2 power-law CSR ({ix.size} nnz) x 262144 x 64 f32, int32 indices, seed 1; "
+                  f"median of {reps} reps after 3 warm-ups, construction excluded",
+        "ms": round(med * 1e3, 4), "reps": reps, **info}}
+    del A, Bc
+    # BASELINE config 1 (the reference's own CPU-runnable case)
+    ip1, ix1, v1, x1 = synthetic.uniform_rows(16384, 16384, 16, seed=0)
+    A1 = torch.sparse_csr_tensor(torch.from_numpy(ip1), torch.from_numpy(ix1), torch.from_numpy(v1.astype(np.float32)),
+                                 size=(16384, 16384))
+    xc = torch.from_numpy(x1.astype(np.float32)).unsqueeze(1)
+    med1, reps1 = time_cpu(lambda: torch.sparse.mm(A1, xc))
+    nbytes1 = ix1.size * 8 + 16385 * 4 + 16384 * 4 * 2
+    res["c1_spmv"] = {"value": round(nbytes1 / med1 / 1e9, 3), "unit": "GB/s", "cores": torch.get_num_threads(),
+                      "kind": "reference", "ms": round(med1 * 1e3, 5), "reps": reps1,
+                      "sample": "torch.sparse.mm(A, x.unsqueeze(1)) on CPU tensors, 16384^2 CSR with 16 distinct "
+                                "uniform columns per row, f32 / int32, seed 0 (whole workload)"}
+    # the C oracle (one core, long double) on the headline workload
+    t0 = time.perf_counter()
+    oracle.spmm(ip, ix, v, B, (n, n))
+    dt = time.perf_counter() - t0
+    res["oracle"] = {"value": round(nbytes / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+                     "sample": f"C oracle (long double) SpMM over the whole headline workload, one run, {dt:.2f} s"}
     return res
 
 
@@ -636,8 +674,17 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline")
         cb = cpu_baseline(r["inputs"])
-        line["cpu_baseline"] = cb["oracle"]
-        line["cpu_torch_sparse"] = cb["torch_sparse"]
+        line["cpu_baseline"] = cb["torch_sparse"]
+        line["cpu_baseline"]["gpu_speedup"] = round(value / cb["torch_sparse"]["value"], 2)
+        line["cpu_oracle_port"] = cb["oracle"]
+        c1 = dict(cb["c1_spmv"])
+        small = line.get("spmv_uniform_16384x16")
+        if small:
+            c1["gpu_graph_us_per_call"] = round(small["graph_ms_per_call"] * 1e3, 3)
+            c1["gpu_speedup"] = round(c1["ms"] / small["graph_ms_per_call"], 2)
+            small["cpu_torch_sparse"] = c1
+        else:
+            line["cpu_c1_spmv_16384x16"] = c1
     print(json.dumps(line), flush=True)
 
 
