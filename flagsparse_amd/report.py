@@ -18,7 +18,8 @@ contract its tests/ci/test_runner_result_format.py pins):
 Each operator's accuracy phase runs its GPU parity tests in a child pytest;
 its performance phase runs the benchmark cases (this package's HIP kernels
 vs torch.sparse, i.e. rocSPARSE) in a child Python, one operator at a time.
-The spreadsheet and HTML renderings of the reference are not produced.
+``result.html`` (:2360-2449) is written beside the summaries; the spreadsheet
+rendering is not produced.
 
 CLI: ``python -m flagsparse_amd.report [--ops spmv_csr,spmm_csr]
 [--phase both|accuracy|performance] [--results-dir DIR] [--quick]
@@ -595,7 +596,164 @@ def write_summary(results, results_dir, env_info):
         w.writeheader()
         for row in rows:
             w.writerow({k: row.get(k, "") for k in _SUMMARY_CSV_HEADERS})
+    write_result_html(ordered, results_dir, env_info)
     return summary
+
+
+# ------------------------------------------------------------- result.html
+_HTML_DTYPES = ("fp16", "fp32", "fp64", "bf16")
+_HTML_STYLE = """<style>
+body { font-family: sans-serif; font-size: 13px; }
+table { border-collapse: collapse; margin-bottom: 12px; }
+th, td { border: 1px solid #bbb; padding: 3px 6px; vertical-align: top; }
+th { background: #eee; }
+pre { max-height: 320px; overflow: auto; background: #f7f7f7; margin: 4px 0; }
+.table-stats { margin: 6px 0; }
+</style>"""
+# filterTable() hides rows whose accuracy / performance status does not match
+# the header selects; sortTable(col, dir) orders rows by a numeric column
+# (dir "reset" restores the operator order).
+_HTML_SCRIPT = """<script>
+function filterTable() {
+  var sel = document.querySelectorAll('select.status-filter');
+  var rows = document.querySelectorAll('#main-table tbody tr'); var shown = 0;
+  rows.forEach(function (r) {
+    var ok = true;
+    sel.forEach(function (s) {
+      var want = s.value; var cell = r.cells[parseInt(s.dataset.col)].querySelector('summary');
+      if (want && cell && cell.textContent.trim() !== want) { ok = false; }
+    });
+    r.style.display = ok ? '' : 'none'; if (ok) { shown += 1; }
+  });
+  document.getElementById('visible-count').textContent = shown;
+}
+function sortTable(col, dir) {
+  var body = document.querySelector('#main-table tbody');
+  var rows = Array.prototype.slice.call(body.rows);
+  rows.sort(function (a, b) {
+    if (dir === 'reset') { return parseInt(a.cells[0].textContent) - parseInt(b.cells[0].textContent); }
+    var x = parseFloat(a.cells[col].textContent), y = parseFloat(b.cells[col].textContent);
+    x = isNaN(x) ? -Infinity : x; y = isNaN(y) ? -Infinity : y;
+    return dir === 'asc' ? x - y : y - x;
+  });
+  rows.forEach(function (r) { body.appendChild(r); });
+}
+</script>"""
+
+
+def _esc(value):
+    import html  # noqa: PLC0415
+
+    return html.escape("" if value is None else str(value))
+
+
+def _html_status(r):
+    return _flaggems_status(r.get("status")) if isinstance(r, dict) else "NotFound"
+
+
+def _html_counts(r):
+    if not isinstance(r, dict) or r.get("phase") == "performance":
+        return ""
+    return f"{r.get('passed', 0)}/{r.get('failed', 0)}/{r.get('skipped', 0)}"
+
+
+def _html_speedups(r):
+    """(mean speedup, {dtype: mean speedup}) of a performance phase."""
+    if not isinstance(r, dict):
+        return None, {}
+    per = {}
+    for row in r.get("records") or []:
+        sp = _to_float(row.get("speedup"))
+        if sp is None:
+            continue
+        per.setdefault(_FLAGGEMS_DTYPES.get(str(row.get("dtype")), str(row.get("dtype"))), []).append(sp)
+    means = {k: statistics.mean(v) for k, v in per.items()}
+    mean = _to_float(r.get("speedup"))
+    if mean is None and means:
+        mean = statistics.mean(means.values())
+    return mean, means
+
+
+def _html_details(phase, r, results_dir):
+    """The phase's data file and log, each in a <pre> block."""
+    if not isinstance(r, dict):
+        return ""
+    parts = []
+    for key in ("data_file", "stdout_log_path"):
+        rel = r.get(key)
+        if not rel:
+            continue
+        path = Path(rel) if Path(str(rel)).is_absolute() else Path(results_dir) / str(rel)
+        try:
+            text = path.read_text(encoding="utf-8", errors="replace")[:20000]
+        except OSError:
+            text = "(not found)"
+        parts.append(f"<div>{_esc(Path(str(rel)).name)}</div><pre>{_esc(text)}</pre>")
+    parts.append(f"<div>{phase}.json</div><pre>{_esc(json.dumps(_flaggems_phase_result({**r, 'phase': phase}), indent=2, default=str))}</pre>")
+    return "".join(parts)
+
+
+def write_result_html(results, results_dir, env_info):
+    """result.html: the environment table and one row per operator with its
+    accuracy / performance status, counts, mean speedup and per-dtype
+    speedups (the report run_flagsparse_pytest.py:2360-2449 writes)."""
+    results_dir = Path(results_dir)
+    ordered = sorted(results, key=lambda r: str(r["operator"]))
+    fg = _flag_gems_env_info(env_info)
+    env_rows = [f"<tr><td>generated</td><td><tt>{_esc(_dt.datetime.now().strftime('%Y-%m-%d %H:%M:%S'))}</tt></td></tr>"]
+    for key, value in (("architecture", fg["architecture"]), ("os", f"{fg['os_name']} {fg['os_release']}"),
+                       ("python", fg["python"]), ("torch", fg["torch"]["version"]),
+                       ("device", fg["torch"]["device_name"]), ("device_count", fg["torch"]["device_count"]),
+                       ("flagsparse_amd", fg["flag_gems"]["version"]),
+                       ("git", (env_info.get("git") or {}).get("commit"))):
+        env_rows.append(f"<tr><td>{_esc(key)}</td><td><tt>{_esc(value)}</tt></td></tr>")
+    statuses = ("Passed", "Failed", "Skipped", "Error", "Timeout", "NotFound")
+
+    def status_select(col):
+        opts = "".join(f'<option value="{s}">{s}</option>' for s in statuses)
+        return (f'<select class="status-filter" data-col="{col}" onchange="filterTable()">'
+                f'<option value="">All</option>{opts}</select>')
+
+    rows = []
+    for i, res in enumerate(ordered, start=1):
+        op = str(res["operator"])
+        acc, perf = res.get("accuracy"), res.get("performance")
+        mean, by_dtype = _html_speedups(perf)
+        dtype_cells = "".join(
+            f"<td>{'' if by_dtype.get(d) is None else f'{by_dtype[d]:.3f}'}</td>" for d in _HTML_DTYPES)
+        note = next((str(r.get("reason")) for r in (acc, perf) if isinstance(r, dict) and r.get("reason")), "")
+        rows.append(
+            f"<tr><td>{i}</td><td>{_esc(op)}</td>"
+            f"<td><details><summary>{_esc(_html_status(acc))}</summary>{_html_details('accuracy', acc, results_dir)}"
+            f"</details></td><td>{_esc(_html_counts(acc))}</td>"
+            f"<td><details><summary>{_esc(_html_status(perf))}</summary>"
+            f"{_html_details('performance', perf, results_dir)}</details></td>"
+            f"<td>{'' if mean is None else f'{mean:.3f}'}</td>{dtype_cells}<td>{_esc(note)}</td></tr>")
+    sort_buttons = "".join(f"<button onclick=\"sortTable(5, '{d}')\">{label}</button>"
+                           for d, label in (("asc", "&#9650;"), ("desc", "&#9660;"), ("reset", "&#8634;")))
+    dtype_heads = "".join(f"<th>{d}</th>" for d in _HTML_DTYPES)
+    text = f"""<!DOCTYPE html>
+<html>
+<head>
+<meta charset="utf-8">
+<title>FlagSparse Test Report</title>
+{_HTML_STYLE}
+</head>
+<body>
+<h3>Test Environment</h3>
+<table><thead><tr><th>Env</th><th>Setting</th></tr></thead>
+<tbody>{"".join(env_rows)}</tbody></table>
+<h3>Test Result</h3>
+<div class="table-stats">Total: <span id="total-count">{len(ordered)}</span> | Showing: <span id="visible-count">{len(ordered)}</span></div>
+<table id="main-table">
+<thead><tr><th>No.</th><th>ID</th><th>AccRes<br>{status_select(2)}</th><th>AccStat(pass/fail/skip)</th>
+<th>PerfRes<br>{status_select(4)}</th><th>OPAverageSpeedUp<br>{sort_buttons}</th>{dtype_heads}<th>Note</th></tr></thead>
+<tbody>{"".join(rows)}</tbody></table>
+{_HTML_SCRIPT}
+</body>
+</html>
+"""
+    (results_dir / "result.html").write_text(text, encoding="utf-8")
 
 
 def _should_fail(results, strict=False):

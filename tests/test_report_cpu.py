@@ -56,6 +56,25 @@ def test_summary_schema(tmp_path):
     assert rows[0]["duration"] == "1.25" and rows[0]["stderr_log_path"] == "spmv_csr/accuracy_stderr.log"
 
 
+def test_result_html(tmp_path):
+    """result.html carries the anchors the reference's runner-format test
+    checks (tests/ci/test_runner_result_format.py:166-177)."""
+    (tmp_path / "spmv_csr").mkdir()
+    (tmp_path / "spmv_csr" / "accuracy_result.json").write_text('{"x::y": "passed"}')
+    perf = _phase("performance", "PASS", row_count=2, speedup=2.5,
+                  records=[{"dtype": "float32", "speedup": "2.0"}, {"dtype": "float64", "speedup": "3.0"}])
+    results = [{"operator": "spmv_csr", "gpu": 0, "customized": True, "labels": [],
+                "accuracy": _phase("accuracy", "PASS", passed=3, failed=0, skipped=1, errors=0, total=4),
+                "performance": perf}]
+    report.write_summary(results, tmp_path, _env())
+    html = (tmp_path / "result.html").read_text(encoding="utf-8")
+    for anchor in ("FlagSparse Test Report", "<h3>Test Environment</h3>", "<h3>Test Result</h3>",
+                   'class="table-stats"', "AccRes<br>", "PerfRes<br>", "OPAverageSpeedUp", "filterTable()",
+                   "sortTable(5, 'asc')", "spmv_csr", "accuracy_result.json", "3/0/1", "2.500", "Test GPU"):
+        assert anchor in html, anchor
+    assert "<td>2.000</td>" in html and "<td>3.000</td>" in html  # fp32 / fp64 columns
+
+
 def test_schema_fixed_on_errors_and_missing_phase():
     acc = report._flaggems_accuracy_result({"status": "CRASH", "errors": 2, "reason": "x", "exit_code": 17})
     perf = report._flaggems_performance_result({"status": "CRASH", "reason": "y", "exit_code": 18})
