@@ -113,19 +113,21 @@ def bench_spmm_power_law(dev, steps, warmup):
 
 
 def bench_gather_probe(dev, n_rows, row_bytes, n_gathers, steps, warmup):
-    """Random whole-row reads from a table of B's footprint (fsp_gather_rows_probe),
-    one per nonzero: the practical ceiling of the SpMM's B-row gathers, timed
-    like the SpMM (HIP events on the launch stream).  Returns (ms, GB/s)."""
-    from flagsparse_amd import _lib  # noqa: PLC0415
+    """Random whole-row reads from a table of B's footprint (fsp_gather_rows_probe
+    of the measurement library), one per nonzero: the practical ceiling of the
+    SpMM's B-row gathers, timed like the SpMM (HIP events on the launch
+    stream).  Returns (ms, GB/s)."""
+    from flagsparse_amd import _bench_lib  # noqa: PLC0415
 
     table = torch.empty(n_rows * row_bytes // 4, dtype=torch.float32, device=dev).normal_()
     sink = torch.zeros(1, dtype=torch.float32, device=dev)
-    lib = _lib.load()
+    lib = _bench_lib.load()
     stream = torch.cuda.current_stream(dev)
+    table_bytes = table.numel() * 4
 
     def run():
-        _lib.check(lib.fsp_gather_rows_probe(table.data_ptr(), n_rows, row_bytes, n_gathers, sink.data_ptr(),
-                                             stream.cuda_stream), "fsp_gather_rows_probe")
+        _bench_lib.check(lib.fsp_gather_rows_probe(table.data_ptr(), table_bytes, n_rows, row_bytes, n_gathers,
+                                                   sink.data_ptr(), stream.cuda_stream), "fsp_gather_rows_probe")
 
     _, ev = time_steps(run, steps, warmup, stream)
     ms = statistics.median(ev)

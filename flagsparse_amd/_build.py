@@ -1,9 +1,12 @@
-"""Builds libflagsparse_hip.so (gfx950) from flagsparse_amd/csrc with hipcc.
+"""Builds libflagsparse_hip.so (gfx950) from flagsparse_amd/csrc with hipcc,
+and the measurement library libflagsparse_bench.so from csrc/bench.
 
 Each .hip translation unit is compiled to an object under _build/ in
-parallel, then linked into flagsparse_amd/libflagsparse_hip.so.  The library
-links libamdhip64.so.7; imported after torch it binds to the HIP runtime torch
-already loaded (same SONAME), so torch's stream handles are valid in it.
+parallel, then linked into flagsparse_amd/libflagsparse_hip.so (the operator
+library: csrc/*.hip) and flagsparse_amd/libflagsparse_bench.so (the probes
+bench.py times: csrc/bench/*.hip).  The libraries link libamdhip64.so.7;
+imported after torch they bind to the HIP runtime torch already loaded (same
+SONAME), so torch's stream handles are valid in them.
 """
 
 import os
@@ -18,6 +21,8 @@ REPO = os.path.dirname(PKG_DIR)
 BUILD_DIR = os.path.join(PKG_DIR, "_build")
 LIB_NAME = "libflagsparse_hip.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
+BENCH_CSRC = os.path.join(CSRC, "bench")
+BENCH_LIB_PATH = os.path.join(PKG_DIR, "libflagsparse_bench.so")
 
 ARCH = os.environ.get("FLAGSPARSE_AMD_ARCH", "gfx950")
 
@@ -46,14 +51,19 @@ def _hipcc():
     raise RuntimeError("hipcc not found: the flagsparse_amd HIP library cannot be built")
 
 
-def sources():
+def sources(directory=CSRC):
     return sorted(
-        os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip")
+        os.path.join(directory, f) for f in os.listdir(directory) if f.endswith(".hip")
     )
 
 
+def _object(src):
+    sub = "bench_" if os.path.dirname(src) == BENCH_CSRC else ""
+    return os.path.join(BUILD_DIR, sub + os.path.basename(src)[:-4] + ".o")
+
+
 def _compile(hipcc, src):
-    obj = os.path.join(BUILD_DIR, os.path.basename(src)[:-4] + ".o")
+    obj = _object(src)
     cmd = [hipcc, *HIP_FLAGS, "-c", src, "-o", obj]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
@@ -63,24 +73,31 @@ def _compile(hipcc, src):
     return obj
 
 
-def build(verbose=True, jobs=None):
-    """Compile every csrc/*.hip for gfx950 and link the shared library."""
-    hipcc = _hipcc()
-    os.makedirs(BUILD_DIR, exist_ok=True)
-    srcs = sources()
-    jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
-    if verbose:
-        print(f"[flagsparse_amd] compiling {len(srcs)} HIP sources for {ARCH}", flush=True)
-    with ThreadPoolExecutor(max_workers=jobs) as pool:
-        objs = list(pool.map(lambda s: _compile(hipcc, s), srcs))
-    tmp = LIB_PATH + ".tmp"
+def _link(hipcc, objs, path):
+    tmp = path + ".tmp"
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{proc.stdout}\n{proc.stderr}")
-    os.replace(tmp, LIB_PATH)
+    os.replace(tmp, path)
+
+
+def build(verbose=True, jobs=None):
+    """Compile every csrc/*.hip and csrc/bench/*.hip for gfx950 and link the
+    operator and measurement libraries."""
+    hipcc = _hipcc()
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    srcs = sources()
+    bench_srcs = sources(BENCH_CSRC)
+    jobs = jobs or min(len(srcs) + len(bench_srcs), max(1, (os.cpu_count() or 4)))
     if verbose:
-        print(f"[flagsparse_amd] built {LIB_PATH}", flush=True)
+        print(f"[flagsparse_amd] compiling {len(srcs) + len(bench_srcs)} HIP sources for {ARCH}", flush=True)
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        objs = list(pool.map(lambda s: _compile(hipcc, s), srcs + bench_srcs))
+    _link(hipcc, objs[: len(srcs)], LIB_PATH)
+    _link(hipcc, objs[len(srcs):], BENCH_LIB_PATH)
+    if verbose:
+        print(f"[flagsparse_amd] built {LIB_PATH} and {BENCH_LIB_PATH}", flush=True)
     return LIB_PATH
 
 

@@ -1,4 +1,5 @@
-// Random-row gather probe (a measurement kernel): n_gathers reads of whole
+// Random-row gather probe (a measurement kernel of libflagsparse_bench.so,
+// not of the operator library): n_gathers reads of whole
 // rows of row_bytes from a table of n_rows rows, the row ids from a hash, so
 // no index stream sits in the dependence chain.  The rate it reaches is the
 // practical ceiling of a kernel whose traffic is such gathers -- the CSR SpMM,
@@ -7,10 +8,24 @@
 //
 // Lane layout: G = row_bytes / 16 lanes read one row with one 16-byte load
 // each, 64 / G rows per wave instruction, 8 instructions in flight per lane.
-#include "fsp_device.hpp"
-#include "fsp_host.hpp"
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "fsp_bench.h"
 
 namespace fsp {
+
+static thread_local char g_bench_error[256];
+
+static int bench_fail(int status, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_bench_error, sizeof(g_bench_error), fmt, ap);
+  va_end(ap);
+  return status;
+}
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {
   x ^= x >> 16;
@@ -60,23 +75,29 @@ __global__ __launch_bounds__(kProbeThreads) void gather_rows_probe_kernel(const 
 
 using namespace fsp;
 
-extern "C" int fsp_gather_rows_probe(const void* table_dev, int64_t n_rows, int32_t row_bytes, int64_t n_gathers,
-                                     void* sink_dev, void* stream) {
+extern "C" const char* fsp_bench_last_error(void) { return g_bench_error; }
+
+extern "C" int fsp_gather_rows_probe(const void* table_dev, int64_t table_bytes, int64_t n_rows, int32_t row_bytes,
+                                     int64_t n_gathers, void* sink_dev, void* stream) {
   if (n_rows <= 0 || n_rows > int64_t(0xffffffff) || n_gathers < 0) {
-    return fail(FSP_ERR_ARG, "fsp_gather_rows_probe: n_rows must be in [1, 2^32) and n_gathers >= 0");
+    return bench_fail(-1, "fsp_gather_rows_probe: n_rows must be in [1, 2^32) and n_gathers >= 0");
   }
   if (row_bytes < 16 || row_bytes > 1024 || (row_bytes & (row_bytes - 1)) != 0) {
-    return fail(FSP_ERR_ARG, "fsp_gather_rows_probe: row_bytes %d must be a power of two in [16, 1024]", row_bytes);
+    return bench_fail(-1, "fsp_gather_rows_probe: row_bytes %d must be a power of two in [16, 1024]", row_bytes);
+  }
+  if (table_bytes < n_rows * row_bytes) {
+    return bench_fail(-1, "fsp_gather_rows_probe: n_rows x row_bytes = %lld exceeds table_bytes = %lld",
+                      (long long)(n_rows * row_bytes), (long long)table_bytes);
   }
   if (table_dev == nullptr || sink_dev == nullptr || reinterpret_cast<uintptr_t>(table_dev) % 16 != 0) {
-    return fail(FSP_ERR_ARG, "fsp_gather_rows_probe: NULL or misaligned (16-byte) table / sink");
+    return bench_fail(-1, "fsp_gather_rows_probe: NULL or misaligned (16-byte) table / sink");
   }
-  if (n_gathers == 0) return FSP_OK;
+  if (n_gathers == 0) return 0;
   const int G = row_bytes / 16;
   const int64_t per_block = int64_t(kProbeThreads / 64) * (64 / G) * kProbeUnroll;
-  int64_t blocks = ceil_div(n_gathers, per_block);
+  int64_t blocks = (n_gathers + per_block - 1) / per_block;
   if (blocks > 2048) blocks = 2048;  // 8 waves per SIMD on 256 CUs; the loop strides the rest
-  const hipStream_t s = as_stream(stream);
+  const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const dim3 grid(static_cast<unsigned>(blocks)), block(kProbeThreads);
   const float4* t = static_cast<const float4*>(table_dev);
   const uint32_t n = static_cast<uint32_t>(n_rows);
@@ -90,5 +111,6 @@ extern "C" int fsp_gather_rows_probe(const void* table_dev, int64_t n_rows, int3
     case 32: hipLaunchKernelGGL(gather_rows_probe_kernel<32>, grid, block, 0, s, t, n, n_gathers, sink); break;
     default: hipLaunchKernelGGL(gather_rows_probe_kernel<64>, grid, block, 0, s, t, n, n_gathers, sink); break;
   }
-  return check_launch("gather_rows_probe");
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : bench_fail(-3, "gather_rows_probe: %s", hipGetErrorString(e));
 }

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSP_ABI_VERSION 11
+#define FSP_ABI_VERSION 12
 #define FSP_MAX_BINS 8
 
 typedef enum fsp_value_type {
@@ -321,18 +321,6 @@ int fsp_batch_pack(int32_t indptr_type, int32_t index_type, int64_t n_mats, cons
                    const void* local_indices_dev, int64_t total_rows, int64_t total_nnz, int32_t out_indptr_type,
                    void* out_indptr_dev, int32_t out_index_type, void* out_indices_dev, uint32_t* err_dev,
                    void* stream);
-
-/* ---- measurement ---------------------------------------------------------
- * No counterpart in the reference: a probe of the gather ceiling that bounds
- * the CSR SpMM.  n_gathers reads of whole rows of row_bytes (a power of two in
- * [16, 1024]) from table_dev, which holds n_rows (1 .. 2^32 - 1) rows of
- * row_bytes and starts on a 16-byte boundary, at row ids from a hash (no
- * index stream).  Nothing is written except, for a sum no ordinary table
- * produces, one float at sink_dev.  Time it on `stream`: n_gathers * row_bytes
- * / time is the practical rate of random row reads of that size from a table
- * of that footprint (the SpMM reads one B row of N values per nonzero). */
-int fsp_gather_rows_probe(const void* table_dev, int64_t n_rows, int32_t row_bytes, int64_t n_gathers,
-                          void* sink_dev, void* stream);
 
 #ifdef __cplusplus
 }

@@ -74,13 +74,19 @@ def test_missing_library_fails_loudly(tmp_path):
         _lib.load(str(tmp_path / "nope.so"))
 
 
-def test_gather_probe_rejects_arguments_without_a_device():
-    lib = _lib.load()
-    assert lib.fsp_gather_rows_probe(None, 64, 256, 10, None, None) == -1  # NULL table
-    assert lib.fsp_gather_rows_probe(None, 64, 24, 10, None, None) == -1
-    assert b"row_bytes" in lib.fsp_last_error()
-    assert lib.fsp_gather_rows_probe(None, 1 << 32, 256, 10, None, None) == -1
-    assert b"n_rows" in lib.fsp_last_error()
+def test_gather_probe_lives_in_the_bench_library():
+    from flagsparse_amd import _bench_lib
+
+    assert not hasattr(ctypes.CDLL(_lib.library_path()), "fsp_gather_rows_probe")
+    lib = _bench_lib.load()
+    assert lib.fsp_gather_rows_probe(None, 1 << 20, 64, 256, 10, None, None) == -1  # NULL table
+    assert lib.fsp_gather_rows_probe(None, 1 << 20, 64, 24, 10, None, None) == -1
+    assert b"row_bytes" in lib.fsp_bench_last_error()
+    assert lib.fsp_gather_rows_probe(None, 1 << 40, 1 << 32, 256, 10, None, None) == -1
+    assert b"n_rows" in lib.fsp_bench_last_error()
+    # a table smaller than n_rows x row_bytes is refused before any launch
+    assert lib.fsp_gather_rows_probe(None, 64 * 256 - 1, 64, 256, 10, None, None) == -1
+    assert b"table_bytes" in lib.fsp_bench_last_error()
 
 
 def test_noalloc_spmm_rejects_oversized_partials_without_a_device():
