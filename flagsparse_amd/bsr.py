@@ -18,7 +18,7 @@ import ctypes
 
 import torch
 
-from . import _lib
+from . import _lib, _plan
 
 BLOCK_DIM = 16
 _BSR_KERNEL_DTYPES = (torch.bfloat16, torch.float16, torch.uint8)  # uint8 = FP8 e4m3 codes
@@ -67,17 +67,17 @@ def _validate_bsr(data, indices, indptr, shape, block_dim):
     data = data.contiguous()
     indices = indices.contiguous()
     indptr = indptr.contiguous()
-    if int(indptr[0].item()) != 0:
+    # every check from one device inspection and one host readback
+    nnzb = int(indices.numel())
+    stats, _ = _plan.inspect_csr(indptr, indices, n_block_rows, nnzb)
+    if stats["indptr_first"] != 0:
         raise ValueError("indptr must start at zero")
-    if int(indptr[-1].item()) != data.shape[0]:
+    if stats["indptr_last"] != data.shape[0]:
         raise ValueError("indptr[-1] must equal nnzb")
-    if indptr.numel() > 1 and torch.any(indptr[1:] < indptr[:-1]).item():
+    if stats["n_decreasing"]:
         raise ValueError("indptr must be non-decreasing")
-    if indices.numel() > 0:
-        min_index = int(indices.min().item())
-        max_index = int(indices.max().item())
-        if min_index < 0 or max_index >= n_block_cols:
-            raise IndexError("indices out of range for n_block_cols")
+    if nnzb > 0 and (stats["min_col"] < 0 or stats["max_col"] >= n_block_cols):
+        raise IndexError("indices out of range for n_block_cols")
     if indices.dtype != torch.int32:
         indices = indices.to(torch.int32)
     return data, indices, indptr, n_rows, n_cols, n_block_rows, n_block_cols
