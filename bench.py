@@ -134,6 +134,76 @@ def bench_gather_probe(dev, n_rows, row_bytes, n_gathers, steps, warmup):
     return ms, n_gathers * row_bytes / (ms * 1e-3) / 1e9
 
 
+def bench_slab_schedule(dev, inputs, steps, configs=((1, 8, -1), (4, 8, 1 << 20), (16, 8, 1 << 20), (16, 8, -1))):
+    """The headline SpMM under the column-slab schedule (fsp_slab_spmm_probe of
+    the measurement library): B swept in S slabs, the workgroups of one XCD
+    passing a phase counter after each slab (spin >= 0) or not (spin < 0).
+    Rows above 256 nonzeros run as 256-nonzero segments; rows are dealt to
+    lane groups longest first.  Reports kernel time (HIP events, median),
+    the error against flagsparse_spmm_csr and whether a bounded wait ran
+    out.  Not part of the default line (a measurement of a rejected design)."""
+    from flagsparse_amd import _bench_lib  # noqa: PLC0415
+
+    ip, ix, v, Bn = inputs
+    n = ip.size - 1
+    data, idx, ptr = synthetic.to_torch(ip, ix, v, dev, torch.float32, torch.int32)
+    B = synthetic.dense(Bn, dev, torch.float32)
+    ref = fs.flagsparse_spmm_csr(data, idx, ptr, B, (n, n))
+    lens = np.diff(ip)
+    seg = 256
+    short = lens <= seg
+    beg, end, slot, seg_row = [ip[:-1][short]], [ip[1:][short]], [np.nonzero(short)[0]], []
+    n_seg = 0
+    for r in np.nonzero(~short)[0]:
+        k = -(-int(lens[r]) // seg)
+        j = np.arange(k)
+        beg.append(ip[r] + lens[r] * j // k)
+        end.append(ip[r] + lens[r] * (j + 1) // k)
+        slot.append(n + n_seg + j)
+        seg_row.append(np.full(k, r))
+        n_seg += k
+    vb, ve, vs = (np.concatenate(a).astype(np.int64) for a in (beg, end, slot))
+    order = np.argsort(-(ve - vb), kind="stable")  # longest first, dealt round-robin to lane groups
+    vb, ve, vs = vb[order], ve[order], vs[order]
+    nv = vb.size
+    vlen = ve - vb
+    vid = np.repeat(np.arange(nv), vlen)
+    pos = np.repeat(vb - (np.cumsum(vlen) - vlen), vlen) + np.arange(vid.size)
+    cols = ix[pos].astype(np.int64)
+    vbeg_t = torch.from_numpy(vb.astype(np.int32)).to(dev)
+    vslot_t = torch.from_numpy(vs.astype(np.int32)).to(dev)
+    seg_row_t = torch.from_numpy(np.concatenate(seg_row)).to(dev) if seg_row else None
+    C = torch.zeros((n, 64), device=dev)  # rows cut into segments are never written: their sum is added below
+    P = torch.empty((max(1, n_seg), 64), device=dev)
+    counters = torch.zeros(8, dtype=torch.int32, device=dev)
+    flag = torch.zeros(1, dtype=torch.int32, device=dev)
+    wgs = torch.cuda.get_device_properties(dev).multi_processor_count
+    stream = torch.cuda.current_stream(dev)
+    lib = _bench_lib.load()
+    out = {"virtual_rows": int(nv), "segments": int(n_seg), "workgroups": wgs}
+    for n_slabs, rpg, spin in configs:
+        cnt = np.bincount(vid * n_slabs + cols * n_slabs // n, minlength=nv * n_slabs).reshape(nv, n_slabs)
+        slab_end = torch.from_numpy(np.cumsum(cnt, axis=1).astype(np.uint16).view(np.int16)).to(dev)
+
+        def run():
+            st = lib.fsp_slab_spmm_probe(rpg, vbeg_t.data_ptr(), slab_end.data_ptr(), vslot_t.data_ptr(),
+                                         idx.data_ptr(), data.data_ptr(), B.data_ptr(), C.data_ptr(), P.data_ptr(), n,
+                                         nv, n_slabs, wgs, counters.data_ptr(), spin, flag.data_ptr(),
+                                         stream.cuda_stream)
+            if st != 0:
+                raise RuntimeError(f"fsp_slab_spmm_probe failed with status {st}")
+
+        flag.zero_()
+        _, ev = time_steps(run, max(3, steps // 2), 1, stream)
+        got = C.clone()
+        if seg_row_t is not None:
+            got.index_add_(0, seg_row_t, P[:n_seg])
+        err = float((got - ref).abs().max() / ref.abs().max())
+        out[f"slabs{n_slabs}_rpg{rpg}_{'sync' if spin >= 0 else 'nosync'}"] = {
+            "ms": round(statistics.median(ev), 4), "rel_err_vs_spmm": err, "wait_ran_out": bool(flag.item())}
+    return out
+
+
 def graph_time_ms(fn, dev, per_graph=64, reps=15):
     """Median device time per call of `fn` replayed from a captured graph of
     `per_graph` calls: the kernels back to back, without host launch gaps."""
@@ -574,8 +644,7 @@ def main():
                     help="also run the 1 GiB copy (the counter calibration kernel) under --no-extras")
     ap.add_argument("--only", default="",
                     help="comma-separated extras to run (copy, spmv_large, spmv_small, spmm_tuned, autograd, bsr, batch, "
-                         "vendor); "
-                         "default: all")
+                         "vendor, slab_schedule); default: all but slab_schedule")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the hot path is a single-GPU kernel); got --gpus {args.gpus}",
@@ -663,6 +732,8 @@ def main():
         line["bsr_65536_b40_n128"] = bench_bsr(dev, args.steps, args.warmup)
     if "batch" in extras:
         line["batch_4096"] = bench_batch(dev, args.steps, args.warmup)
+    if "slab_schedule" in extras:
+        line["spmm_slab_schedule"] = bench_slab_schedule(dev, r["inputs"], args.steps)
     if "vendor" in extras:
         vend = bench_vendor(dev, args.steps, args.warmup, r["inputs"])
         vs = vend.get("spmm_power_law_262144_n64", {}).get("event_ms_median")

@@ -10,8 +10,11 @@ import torch  # noqa: F401  (bind to the HIP runtime torch loaded)
 
 from ._build import BENCH_LIB_PATH
 
+_I32, _VP = ctypes.c_int32, ctypes.c_void_p
+
 SIGNATURES = {
     "fsp_bench_last_error": (ctypes.c_char_p, []),
+    "fsp_slab_spmm_probe": (_I32, [_I32] + [_VP] * 8 + [_I32] * 4 + [_VP, _I32, _VP, _VP]),
     "fsp_gather_rows_probe": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
                                                ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
 }

@@ -1,6 +1,7 @@
 /*
  * fsp_bench.h -- C ABI of libflagsparse_bench.so, the measurement kernels
- * bench.py times beside the operators.  Not part of the operator library
+ * bench.py times beside the operators (the SpMM's gather ceiling and the
+ * column-slab schedule it was measured against).  Not part of the operator library
  * (libflagsparse_hip.so) and not a counterpart of anything in the reference.
  *
  * Return value: 0 or -1 (bad arguments) / -3 (launch failed);
@@ -27,6 +28,25 @@ const char* fsp_bench_last_error(void);
  * B row of N values per nonzero). */
 int fsp_gather_rows_probe(const void* table_dev, int64_t table_bytes, int64_t n_rows, int32_t row_bytes,
                           int64_t n_gathers, void* sink_dev, void* stream);
+
+/* Column-slab schedule probe of the CSR SpMM (N = 64, float32): C = A B
+ * computed slab by slab over n_slabs column ranges of A (row ranges of B),
+ * each lane group keeping rows_per_group (4 or 8) virtual rows' accumulators
+ * in registers across the slabs.  A virtual row v covers nonzeros
+ * [vbeg[v], vbeg[v] + slab_end[v][n_slabs - 1]) of indices / data (sorted
+ * columns, at most 65535 nonzeros); slab_end[v][k] (uint16, row-major
+ * [n_virtual][n_slabs]) is the end offset of slab k inside it; vslot[v] is
+ * its output row: < n_rows writes C[vslot] (row-major [n_rows, 64]), else
+ * partials[vslot - n_rows] ([*, 64]).  B is row-major [K, 64].  spin_limit
+ * >= 0: after every slab the workgroups with equal blockIdx % 8 wait for each
+ * other (counters_dev: 8 int32, zeroed by the call; at most spin_limit polls,
+ * then *flag_dev |= 1); spin_limit < 0: no waits.  n_workgroups 1024-thread
+ * workgroups, launched cooperatively (every one resident, or -3). */
+int fsp_slab_spmm_probe(int32_t rows_per_group, const int32_t* vbeg_dev, const uint16_t* slab_end_dev,
+                        const int32_t* vslot_dev, const int32_t* indices_dev, const float* data_dev,
+                        const float* B_dev, float* C_dev, float* partials_dev, int32_t n_rows,
+                        int32_t n_virtual, int32_t n_slabs, int32_t n_workgroups, int32_t* counters_dev,
+                        int32_t spin_limit, int32_t* flag_dev, void* stream);
 
 #ifdef __cplusplus
 }
