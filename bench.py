@@ -400,11 +400,50 @@ def bench_spmv_large(dev, steps, warmup):
     med = statistics.median(ev)
     nnz = m * 16
     nbytes = nnz * 8 + (m + 1) * 4 + n * 4 + m * 4
+    gbs = nbytes / (med * 1e-3) / 1e9
+    probe = bench_read_probe(dev, idx, data, steps, warmup)
+    probe["spmv_frac_of_best"] = round(gbs / probe["best_gbs"], 4)
     del ptr, idx, data, x, y, handle
     return {"rows": m, "cols": n, "nnz": nnz, "bytes": nbytes, "event_ms_median": round(med, 5),
             "tuned": {k: v for k, v in p.__dict__.items()},
             "traffic": measured_traffic(SPMV_TRAFFIC_SUMMARY) if p.variant == 4 else None,
-            "gbs": round(nbytes / (med * 1e-3) / 1e9, 2), "hbm_frac": round(nbytes / (med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            "gbs": round(gbs, 2), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "read_probe": probe}
+
+
+def bench_read_probe(dev, a, b, steps, warmup):
+    """Streaming reads of the SpMV's column-id and value arrays themselves
+    (fsp_stream_read_probe of the measurement library: 16-byte nontemporal
+    chunks, both arrays in lock step, 1024-thread workgroups): the practical
+    read rate of the index / value stream pattern, with the SpMV's
+    persistent grid (two workgroups per CU, contiguous ranges) and with a
+    grid-stride one; also one array alone."""
+    from flagsparse_amd import _bench_lib  # noqa: PLC0415
+
+    lib = _bench_lib.load()
+    sink = torch.zeros(1, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    chunks = a.numel() * a.element_size() // 16
+    out = {"arrays_bytes": 2 * chunks * 16}
+    best = 0.0
+    for name, n_arrays, wgs, contiguous in (("two_arrays_2wg_per_cu_contiguous", 2, 2 * cus, 1),
+                                            ("two_arrays_2wg_per_cu_strided", 2, 2 * cus, 0),
+                                            ("two_arrays_8wg_per_cu_strided", 2, 8 * cus, 0),
+                                            ("one_array_8wg_per_cu_strided", 1, 8 * cus, 0)):
+        def run():
+            st = lib.fsp_stream_read_probe(a.data_ptr(), b.data_ptr(), chunks, n_arrays, wgs, contiguous,
+                                           sink.data_ptr(), stream.cuda_stream)
+            if st != 0:
+                raise RuntimeError(f"fsp_stream_read_probe failed with status {st}")
+
+        _, ev = time_steps(run, steps, warmup, stream)
+        ms = statistics.median(ev)
+        g = n_arrays * chunks * 16 / (ms * 1e-3) / 1e9
+        out[name] = {"ms": round(ms, 5), "gbs": round(g, 1)}
+        if n_arrays == 2:
+            best = max(best, g)
+    out["best_gbs"] = round(best, 1)
+    return out
 
 
 def bench_vendor(dev, steps, warmup, inputs):

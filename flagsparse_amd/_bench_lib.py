@@ -15,6 +15,7 @@ _I32, _VP = ctypes.c_int32, ctypes.c_void_p
 SIGNATURES = {
     "fsp_bench_last_error": (ctypes.c_char_p, []),
     "fsp_slab_spmm_probe": (_I32, [_I32] + [_VP] * 8 + [_I32] * 4 + [_VP, _I32, _VP, _VP]),
+    "fsp_stream_read_probe": (_I32, [_VP, _VP, ctypes.c_int64, _I32, _I32, _I32, _VP, _VP]),
     "fsp_gather_rows_probe": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
                                                ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
 }

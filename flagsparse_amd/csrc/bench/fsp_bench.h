@@ -48,6 +48,15 @@ int fsp_slab_spmm_probe(int32_t rows_per_group, const int32_t* vbeg_dev, const u
                         int32_t n_virtual, int32_t n_slabs, int32_t n_workgroups, int32_t* counters_dev,
                         int32_t spin_limit, int32_t* flag_dev, void* stream);
 
+/* Streaming-read probe: n_arrays (1 or 2) device arrays (16-byte aligned;
+ * b_dev ignored for one) of n_chunks 16-byte chunks each, read in lock step
+ * (chunk i of every array by one lane, nontemporal) by n_workgroups
+ * 1024-thread workgroups, each a contiguous range of chunks (contiguous = 1)
+ * or chunks b, b + grid, ... (0).  Writes nothing but, for a sum no
+ * ordinary data produces, one float at sink_dev. */
+int fsp_stream_read_probe(const void* a_dev, const void* b_dev, int64_t n_chunks, int32_t n_arrays,
+                          int32_t n_workgroups, int32_t contiguous, void* sink_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
