@@ -346,6 +346,45 @@ def bench_autograd_bf16_vs_fp32(dev, steps, warmup):
     return out
 
 
+def bench_trans_forms(dev, steps, warmup, inputs):
+    """The two forms of the transposed SpMM A^T B against the forward A B:
+    values gathered into CSC order by fsp_gather and then the plain kernel
+    over the CSC (the library's form), and the kernel that loads
+    data[perm[q]] itself.  The 262144^2 power-law matrix at N = 64 and the
+    200000-node graph at N = 128, f32 and bf16; the forms are checked bitwise
+    equal."""
+    from flagsparse_amd._csc import gather_values  # noqa: PLC0415
+
+    stream = torch.cuda.current_stream(dev)
+    out = {}
+    ip, ix, v, _ = inputs
+    cases = [("power_law_262144_n64", ip, ix, v, 262144, 64)]
+    gip, gix, gv = synthetic.power_law_graph(200000, 2.3, 14, seed=3)
+    cases.append(("graph_200000_n128", gip, gix, gv, 200000, 128))
+    for name, ip_, ix_, v_, n, width in cases:
+        for dtype, dname in ((torch.float32, "f32"), (torch.bfloat16, "bf16")):
+            data, idx, ptr = synthetic.to_torch(ip_, ix_, v_, dev, dtype, torch.int32)
+            h = CsrHandle(idx, ptr, (n, n))
+            B = torch.randn(n, width, device=dev).to(dtype)
+            C_g = torch.empty((n, width), dtype=dtype, device=dev)
+            C_p = torch.empty((n, width), dtype=dtype, device=dev)
+            perm = h.csc()[2]
+            res = {}
+            for key, fn in (("fwd", lambda: h.spmm(data, B, out=C_g)),
+                            ("trans_values_via_perm", lambda: h.spmm(data, B, transpose=True, out=C_p,
+                                                                     values_via_perm=True)),
+                            ("gather_only", lambda: gather_values(data, perm)),
+                            ("trans_gathered", lambda: h.spmm(data, B, transpose=True, out=C_g))):
+                _, ev = time_steps(fn, steps, warmup, stream)
+                res[key + "_ms"] = round(statistics.median(ev), 5)
+            torch.cuda.synchronize()
+            res["bitwise_equal"] = bool(torch.equal(C_g, C_p))
+            res["gain"] = round(res["trans_values_via_perm_ms"] / res["trans_gathered_ms"], 3)
+            out[f"{name}_{dname}"] = res
+            log(f"trans forms {name} {dname}: {res}")
+    return out
+
+
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "spmm_bins_kernel_power_law_262144_n64_pmc.md")
 SPMV_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "spmv_vec_kernel_uniform_16m_x16_pmc.md")
 
@@ -683,7 +722,7 @@ def main():
                     help="also run the 1 GiB copy (the counter calibration kernel) under --no-extras")
     ap.add_argument("--only", default="",
                     help="comma-separated extras to run (copy, spmv_large, spmv_small, spmm_tuned, autograd, bsr, batch, "
-                         "vendor, slab_schedule); default: all but slab_schedule")
+                         "vendor, slab_schedule, trans_forms); default: all but slab_schedule and trans_forms")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the hot path is a single-GPU kernel); got --gpus {args.gpus}",
@@ -773,6 +812,8 @@ def main():
         line["batch_4096"] = bench_batch(dev, args.steps, args.warmup)
     if "slab_schedule" in extras:
         line["spmm_slab_schedule"] = bench_slab_schedule(dev, r["inputs"], args.steps)
+    if "trans_forms" in extras:
+        line["spmm_trans_forms"] = bench_trans_forms(dev, args.steps, args.warmup, r["inputs"])
     if "vendor" in extras:
         vend = bench_vendor(dev, args.steps, args.warmup, r["inputs"])
         vs = vend.get("spmm_power_law_262144_n64", {}).get("event_ms_median")

@@ -2,16 +2,19 @@
 
 A ``CsrHandle`` owns (references to) indptr / indices and caches, per kernel
 configuration, the row-bin plans of the CSR and -- on first transposed use --
-the stable CSC permutation (colptr, rowind, perm) with its own plans.  The
-transposed kernels read values through ``perm`` (data[perm[q]]), so a handle
-stays valid while the values change (training) and no transposed copy of A is
-ever materialised.  All launches go through the C ABI of libflagsparse_hip.so.
+the stable CSC permutation (colptr, rowind, perm) with its own plans.  Only
+the index structure is cached: a transposed SpMM gathers the values into CSC
+order on every call (``fsp_gather`` into a buffer of the caching allocator)
+and runs the plain kernel over the CSC, so a handle stays valid while the
+values change (training).  The transposed SpMV reads them through ``perm``
+(data[perm[q]]) inside its kernel.  All launches go through the C ABI of
+libflagsparse_hip.so.
 """
 
 import torch
 
 from . import _lib, _plan
-from ._csc import csr_to_csc_perm
+from ._csc import csr_to_csc_perm, gather_values
 
 
 def _acc_dtype(dtype):
@@ -54,10 +57,17 @@ class CsrHandle:
         return p
 
     # --------------------------------------------------------------- kernels
-    def spmm(self, data, B, transpose=False, out=None, lanes_n=None, config=None, max_blocks=0, xcd_groups=0):
+    def spmm(self, data, B, transpose=False, out=None, lanes_n=None, config=None, max_blocks=0, xcd_groups=0,
+             values_via_perm=False):
         """C = A B (transpose=False) or A^T B (transpose=True); B row-major.
         max_blocks > 0: persistent launch of at most that many workgroups;
-        xcd_groups > 1: XCD-region tile mapping (see _plan.CsrPlan.struct)."""
+        xcd_groups > 1: XCD-region tile mapping (see _plan.CsrPlan.struct).
+        transpose: the values are first gathered into CSC order (one pass of
+        nnz random reads); the kernel then streams them like a plain CSR.  A
+        kernel that loads data[perm[q]] itself puts that random read in every
+        nonzero's dependent chain and measures 10-19 % slower (262144^2 power
+        law at N = 64 and the 200000-node graph at N = 128, f32 and bf16).
+        values_via_perm=True keeps that form; both are bitwise equal."""
         n_out = self.n_cols if transpose else self.n_rows
         n_dense = int(B.shape[1])
         device = B.device
@@ -73,6 +83,8 @@ class CsrHandle:
             colptr, rowind, perm = self.csc()
             plan = self.csc_plan(cfg)
             ptr_t, idx_t, perm_t = colptr, rowind, perm
+            if not values_via_perm:
+                data, perm_t = gather_values(data, perm), None
         else:
             plan = self.plan(cfg)
             ptr_t, idx_t, perm_t = self.indptr, self.indices, None
@@ -104,7 +116,10 @@ class CsrHandle:
         )
         return C
 
-    def spmv(self, data, x, transpose=False, out=None, config=None, max_blocks=0, xcd_groups=0, variant=0):
+    def spmv(self, data, x, transpose=False, out=None, config=None, max_blocks=0, xcd_groups=0, variant=0,
+             values_via_perm=True):
+        """y = A x or A^T x.  transpose with values_via_perm=False gathers the
+        values into CSC order first, as the transposed SpMM does."""
         n_out = self.n_cols if transpose else self.n_rows
         y = out if out is not None else torch.empty(n_out, dtype=data.dtype, device=x.device)
         if n_out == 0:
@@ -114,6 +129,8 @@ class CsrHandle:
             colptr, rowind, perm = self.csc()
             plan = self.csc_plan(cfg)
             ptr_t, idx_t, perm_t = colptr, rowind, perm
+            if not values_via_perm and self.nnz:
+                data, perm_t = gather_values(data, perm), None
         else:
             plan = self.plan(cfg)
             ptr_t, idx_t, perm_t = self.indptr, self.indices, None

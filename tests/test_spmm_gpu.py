@@ -10,10 +10,12 @@ layout, bf16/f16 storage and bitwise determinism.
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 import torch
 
 import flagsparse_amd as fs
 from flagsparse_amd import synthetic
+from flagsparse_amd._matrix import CsrHandle
 from oracle import oracle
 
 from ._helpers import REL_TOL, assert_rel, device_csr, long_row_matrix, np_dtype_round
@@ -266,6 +268,29 @@ def test_spmm_bitwise_equal_across_op_entry_points(device):
     C_trans = fs.flagsparse_spmm_csr(data, idx, ptr, B, (200, 150), op="trans")
     C_non = fs.flagsparse_spmm_csr(dt, it, pt, B, shape_t)
     assert torch.equal(C_trans, C_non)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.bfloat16, torch.float16],
+                         ids=["f32", "f64", "bf16", "f16"])
+@pytest.mark.parametrize("index_dtype", [torch.int32, torch.int64], ids=["i32", "i64"])
+def test_spmm_trans_gathered_values_equal_values_via_perm(device, dtype, index_dtype):
+    # A^T B with values gathered into CSC order first (the default) against the
+    # kernel that loads data[perm[q]] itself; the long rows of A^T (columns 5,
+    # 900, 2047 of A, 3000 nonzeros each) take the segmented path
+    ip_t, ix_t, v_t = long_row_matrix(2048, 3000, [5, 900, 2047], 3000, 4, seed=12)
+    At = sp.csr_matrix((v_t, ix_t, ip_t), shape=(2048, 3000))
+    A = At.T.tocsr()
+    A.sort_indices()
+    data, idx, ptr = device_csr(A.indptr.astype(np.int64), A.indices.astype(np.int64), A.data, device, dtype,
+                                index_dtype)
+    h = CsrHandle(idx, ptr, A.shape)
+    B = torch.randn(3000, 96, device=device).to(dtype)
+    C_gather = h.spmm(data, B, transpose=True)
+    C_perm = h.spmm(data, B, transpose=True, values_via_perm=True)
+    assert torch.equal(C_gather, C_perm)
+    ref = oracle.spmm(A.indptr.astype(np.int64), A.indices.astype(np.int64), np_dtype_round(A.data, dtype),
+                      np_dtype_round(B.double().cpu().numpy(), dtype), A.shape, "trans")
+    assert_rel(C_gather, ref, dtype, "trans gathered")
 
 
 @pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16], ids=["bf16", "f16"])
