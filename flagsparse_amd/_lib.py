@@ -16,7 +16,7 @@ from ._build import LIB_PATH
 FSP_F32, FSP_F64, FSP_F16, FSP_BF16, FSP_F8E4M3 = 0, 1, 2, 3, 4
 FSP_I32, FSP_I64 = 0, 1
 FSP_MAX_BINS = 8
-ABI_VERSION = 12
+ABI_VERSION = 13
 NOALLOC_SCRATCH_BYTES = 16 << 20
 
 _VALUE_CODES = {
@@ -59,6 +59,8 @@ class FspCsrPlan(ctypes.Structure):
         ("max_blocks", _c_i64),
         ("xcd_groups", _c_i64),
         ("variant", _c_i64),
+        ("row_len", _c_i64),
+        ("indptr_base", _c_i64),
     ]
 
 

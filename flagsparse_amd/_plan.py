@@ -162,6 +162,8 @@ class CsrPlan:
     long_seg_ptr: torch.Tensor
     n_segments: int
     n_long_rows: int
+    row_len: int = 0  # > 0: every row has exactly this many nonzeros
+    indptr_base: int = 0  # indptr[0] when row_len > 0
 
     @property
     def bin_sizes(self):
@@ -203,6 +205,8 @@ class CsrPlan:
             off += count
         st.n_segments = self.n_segments
         st.n_long_rows = self.n_long_rows
+        st.row_len = self.row_len
+        st.indptr_base = self.indptr_base
         if self.n_segments:
             st.seg_row_dev = self.seg_row.data_ptr()
             st.seg_begin_dev = self.seg_begin.data_ptr()
@@ -236,12 +240,31 @@ def _launch_row_bins(indptr, n_rows, config, offsets):
     return rows
 
 
+def _row_len_if_constant(n_rows, first, last, max_len):
+    """The common row length when every row has max_len nonzeros (n_rows
+    rows of at most max_len summing to last - first), else 0."""
+    if n_rows > 0 and max_len > 0 and max_len * n_rows == last - first:
+        return int(max_len)
+    return 0
+
+
 def build_plan(indptr, n_rows, config):
     """Bin the rows described by `indptr` (device tensor, int32/int64)."""
     n_rows = int(n_rows)
-    offsets = torch.empty(len(config.uppers) + 2, dtype=torch.int64, device=indptr.device)
-    rows = _launch_row_bins(indptr, n_rows, config, offsets)
-    return _finish_plan(indptr, n_rows, config, rows, offsets.cpu().tolist())  # host sync (prepare time only)
+    nb = len(config.uppers) + 1
+    meta = torch.empty(nb + 1 + 3, dtype=torch.int64, device=indptr.device)
+    rows = _launch_row_bins(indptr, n_rows, config, meta[: nb + 1])
+    if n_rows > 0:
+        ip = indptr[: n_rows + 1]
+        meta[nb + 1] = ip[0]
+        meta[nb + 2] = ip[n_rows]
+        meta[nb + 3] = (ip[1:] - ip[:-1]).max()
+    else:
+        meta[nb + 1:] = 0
+    host = meta.cpu().tolist()  # host sync (prepare time only)
+    first, last, max_len = host[nb + 1:]
+    return _finish_plan(indptr, n_rows, config, rows, host[: nb + 1],
+                        _row_len_if_constant(n_rows, first, last, max_len), first)
 
 
 def inspect_csr(indptr, indices, n_rows, nnz, config=None):
@@ -273,12 +296,15 @@ def inspect_csr(indptr, indices, n_rows, nnz, config=None):
     }
     plan = None
     if config is not None and host[0] == 0 and host[1] == nnz and host[2] == 0:
-        plan = _finish_plan(indptr, n_rows, config, rows, host[6:])
+        plan = _finish_plan(indptr, n_rows, config, rows, host[6:],
+                            _row_len_if_constant(n_rows, host[0], host[1], stats["max_row"]), host[0])
     return stats, plan
 
 
-def _finish_plan(indptr, n_rows, config, rows, off):
-    """The plan from the bin offsets `off` (host list) of a valid CSR."""
+def _finish_plan(indptr, n_rows, config, rows, off, row_len=0, indptr_base=0):
+    """The plan from the bin offsets `off` (host list) of a valid CSR.
+    row_len > 0 (every row that long, indptr[i] = indptr_base + i * row_len)
+    is kept for a single-bin plan: its kernels then compute row bounds."""
     device = indptr.device
     nb = len(config.uppers) + 1
     counts = [off[i + 1] - off[i] for i in range(nb)]
@@ -327,4 +353,6 @@ def _finish_plan(indptr, n_rows, config, rows, off):
         long_seg_ptr=long_seg_ptr,
         n_segments=n_seg,
         n_long_rows=int(n_long),
+        row_len=int(row_len) if identity else 0,
+        indptr_base=int(indptr_base) if identity and row_len else 0,
     )

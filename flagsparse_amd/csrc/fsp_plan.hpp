@@ -43,6 +43,8 @@ struct launch_table {
   const int32_t* seg_row;
   const int64_t* seg_begin;
   const int64_t* seg_end;
+  int64_t row_len;      // > 0: rows of exactly this length (bounds computed, not loaded)
+  int64_t indptr_base;  // indptr[0] when row_len > 0
 };
 
 // Tiles of one workgroup: first, first + step, ... while < end.
@@ -95,8 +97,13 @@ __device__ __forceinline__ work_item load_item(const launch_table& t, const laun
     w.out = rr;
   } else {
     w.row = lb.rows ? static_cast<int64_t>(lb.rows[rr]) : rr;
-    w.begin = static_cast<int64_t>(indptr[w.row]);
-    w.end = static_cast<int64_t>(indptr[w.row + 1]);
+    if (t.row_len > 0) {  // uniform over the launch: no dependent row-pointer load
+      w.begin = t.indptr_base + w.row * t.row_len;
+      w.end = w.begin + t.row_len;
+    } else {
+      w.begin = static_cast<int64_t>(indptr[w.row]);
+      w.end = static_cast<int64_t>(indptr[w.row + 1]);
+    }
     w.out = w.row;
   }
   return w;
@@ -170,6 +177,8 @@ static int64_t build_launch_table(const fsp_csr_plan* plan, int seg_lanes_log2, 
   t->seg_row = plan->seg_row_dev;
   t->seg_begin = plan->seg_begin_dev;
   t->seg_end = plan->seg_end_dev;
+  t->row_len = plan->row_len;
+  t->indptr_base = plan->indptr_base;
   return blocks;
 }
 
@@ -190,6 +199,10 @@ static inline int check_plan(const fsp_csr_plan* plan, int64_t n_rows) {
                 (long long)total, (long long)plan->n_long_rows, (long long)n_rows);
   }
   if (plan->n_segments < 0 || plan->n_long_rows < 0) return fail(FSP_ERR_ARG, "plan: negative segment counts");
+  if (plan->row_len < 0 || plan->indptr_base < 0) {
+    return fail(FSP_ERR_ARG, "plan: row_len %lld / indptr_base %lld negative", (long long)plan->row_len,
+                (long long)plan->indptr_base);
+  }
   if (plan->variant < 0 || plan->variant > 4) return fail(FSP_ERR_ARG, "plan: unknown variant %lld", (long long)plan->variant);
   if (plan->xcd_groups < 0 || plan->xcd_groups > 64) {
     return fail(FSP_ERR_ARG, "plan: xcd_groups %lld outside [0, 64]", (long long)plan->xcd_groups);

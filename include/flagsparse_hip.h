@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSP_ABI_VERSION 12
+#define FSP_ABI_VERSION 13
 #define FSP_MAX_BINS 8
 
 typedef enum fsp_value_type {
@@ -91,6 +91,12 @@ typedef struct fsp_csr_plan {
                                      perm) streams read as 4-element chunks
                                      (arrays aligned to 4 elements, unroll 4
                                      or 8); other kernels ignore it */
+  int64_t row_len;                /* > 0: every row has exactly row_len nonzeros,
+                                     indptr[i] = indptr_base + i * row_len, and
+                                     the SpMV kernels compute row bounds instead
+                                     of loading them; 0: rows vary (bounds are
+                                     loaded).  The caller vouches for it */
+  int64_t indptr_base;            /* indptr[0] when row_len > 0 */
 } fsp_csr_plan;
 
 /* ---- library ------------------------------------------------------------ */

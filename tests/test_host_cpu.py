@@ -203,3 +203,13 @@ def test_availability_queries_and_registries_need_no_device():
     assert fs.resolve_spmm_coo_algorithm("rowrun", 0, torch.float64).display_name == "COORowRun"
     with pytest.raises(ValueError, match="op must be one of"):
         fs.list_spmm_coo_algorithms(op="sideways")
+
+
+def test_row_len_if_constant():
+    from flagsparse_amd._plan import _row_len_if_constant
+
+    assert _row_len_if_constant(4, 0, 64, 16) == 16
+    assert _row_len_if_constant(4, 40, 104, 16) == 16  # indptr starting at 40
+    assert _row_len_if_constant(4, 0, 63, 16) == 0  # one row shorter
+    assert _row_len_if_constant(0, 0, 0, 0) == 0
+    assert _row_len_if_constant(5, 0, 0, 0) == 0  # empty rows only

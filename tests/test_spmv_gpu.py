@@ -380,3 +380,55 @@ def test_spmv_vector_chunk_variant_rejects_misaligned_and_unroll(device):
     # the untuned entry point picks the plain LDS loop for such a view
     assert _plan.spmv_default_variant(700, 4, 1 << 20, _plan.chunk_aligned(idx, shifted)) == 2
     assert _plan.spmv_default_variant(700, 4, 1 << 20, _plan.chunk_aligned(idx, data)) == 4
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.bfloat16], ids=["f32", "f64", "bf16"])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("base", [0, 40])
+def test_spmv_constant_row_length_computes_bounds(device, dtype, variant, base):
+    """Rows that all hold 16 nonzeros: the plan keeps row_len = 16 and the
+    kernels compute each row's bounds instead of loading them.  Bitwise equal
+    to the same plan with row_len cleared (bounds loaded from indptr), on
+    every loop variant, also for an indptr that starts at 40 (the index and
+    value arrays then begin 40 elements in)."""
+    import dataclasses
+
+    from flagsparse_amd import _plan
+    from flagsparse_amd._matrix import CsrHandle
+
+    n_rows, n_cols = 5003, 4096
+    ip, ix, v, x = synthetic.uniform_rows(n_rows, n_cols, 16, seed=base + variant)
+    rng = np.random.default_rng(base)
+    ip_b = ip + base
+    ix_b = np.concatenate([rng.integers(0, n_cols, base), ix])
+    v_b = np.concatenate([rng.uniform(-1, 1, base), v])
+    data, idx, ptr = device_csr(ip_b, ix_b, v_b, device, dtype)
+    xt = synthetic.dense(x, device, dtype)
+    h = CsrHandle(idx, ptr, (n_rows, n_cols))
+    cfg = _plan.spmv_config(unroll=4)
+    plan = h.plan(cfg)
+    assert plan.identity and plan.row_len == 16 and plan.indptr_base == base
+    got = h.spmv(data, xt, config=cfg, variant=variant)
+    h._plans[cfg] = dataclasses.replace(plan, row_len=0, indptr_base=0)
+    loaded = h.spmv(data, xt, config=cfg, variant=variant)
+    assert torch.equal(got, loaded)
+    gold = oracle.spmv(ip, ix, np_dtype_round(v, dtype), np_dtype_round(x, dtype), (n_rows, n_cols))
+    assert_rel(got, gold, dtype, f"row_len variant {variant}")
+
+
+def test_plan_row_len_only_for_constant_rows(device):
+    from flagsparse_amd import _plan
+
+    ip, ix, v, _ = synthetic.uniform_rows(4096, 4096, 16, seed=0)
+    d, i, p = device_csr(ip, ix, v, device, torch.float32, torch.int64, torch.int64)
+    stats, plan = _plan.inspect_csr(p, i, 4096, ix.size, _plan.spmv_config())
+    assert plan.row_len == 16 and plan.indptr_base == 0
+    assert plan.struct().row_len == 16
+    ip2 = ip.copy()
+    ip2[100] += 1  # rows 99 and 100: 17 and 15 nonzeros
+    d, i, p = device_csr(ip2, ix, v, device, torch.float32)
+    assert _plan.build_plan(p, 4096, _plan.spmv_config()).row_len == 0
+    assert _plan.inspect_csr(p, i, 4096, ix.size, _plan.spmv_config())[1].row_len == 0
+    # empty rows only: nothing to compute
+    p0 = torch.zeros(11, dtype=torch.int32, device=device)
+    assert _plan.build_plan(p0, 10, _plan.spmv_config()).row_len == 0
