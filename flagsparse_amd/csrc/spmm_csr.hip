@@ -174,22 +174,27 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
 }
 
 // C[long_rows[i], :] = sum of the row's segment partials, in segment order.
-// One thread per (long row, column); a workgroup covers 256 consecutive
-// (row, column) pairs of the flattened [n_long, n_dense] index space.
+// One wave per long row (a grid-stride loop over rows): the row's segment
+// range and output row are wave-uniform loads, lane l takes columns l,
+// l + 64, ..., and each segment's partial row is one coalesced read whose
+// address does not depend on the previous one.
 template <typename T>
 __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_long_reduce_kernel(
     const typename acc_of<T>::type* __restrict__ partials, const int32_t* __restrict__ long_rows,
     const int64_t* __restrict__ seg_ptr, int64_t n_long, int64_t n_dense, T* __restrict__ C, int64_t ldc) {
   using A = typename acc_of<T>::type;
-  const int64_t total = n_long * n_dense;
-  for (int64_t t = static_cast<int64_t>(blockIdx.x) * FSP_WG_THREADS + threadIdx.x; t < total;
-       t += static_cast<int64_t>(gridDim.x) * FSP_WG_THREADS) {
-    const int64_t i = t / n_dense;
-    const int64_t col = t - i * n_dense;
+  const int lane = threadIdx.x & 63;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * FSP_WAVES + (threadIdx.x >> 6); i < n_long;
+       i += static_cast<int64_t>(gridDim.x) * FSP_WAVES) {
     const int64_t s0 = seg_ptr[i], s1 = seg_ptr[i + 1];
-    A acc = A(0);
-    for (int64_t sg = s0; sg < s1; ++sg) acc += partials[sg * n_dense + col];
-    C[static_cast<int64_t>(long_rows[i]) * ldc + col] = static_cast<T>(acc);
+    T* __restrict__ c_row = C + static_cast<int64_t>(long_rows[i]) * ldc;
+    for (int64_t col = lane; col < n_dense; col += 64) {
+      const A* __restrict__ p = partials + s0 * n_dense + col;
+      A acc = A(0);
+#pragma unroll 4
+      for (int64_t sg = s0; sg < s1; ++sg, p += n_dense) acc += *p;
+      c_row[col] = static_cast<T>(acc);
+    }
   }
 }
 
@@ -276,7 +281,7 @@ static int launch_spmm(const spmm_args& x, const fsp_csr_plan* plan, hipStream_t
     if (st) return st;
   }
   if (plan->n_long_rows > 0) {
-    int64_t g = ceil_div(plan->n_long_rows * x.n_dense, FSP_WG_THREADS);
+    int64_t g = ceil_div(plan->n_long_rows, FSP_WAVES);
     if (g > 65536) g = 65536;
     hipLaunchKernelGGL(spmm_long_reduce_kernel<T>, dim3(static_cast<unsigned>(g)), dim3(FSP_WG_THREADS), 0, s,
                        partials, plan->long_rows_dev, plan->long_seg_ptr_dev, plan->n_long_rows, x.n_dense,
