@@ -261,7 +261,13 @@ __device__ __forceinline__ chunk4<E> load_chunk_tail(const E* p, int64_t q, int6
 
 // Rows of one bin, L lanes per row (a compile-time constant, so the row's
 // group sum is a fixed DPP sequence and the chunk strides are immediates).
-template <int L, typename T, typename IT, typename PT, typename QT, int CHUNKS, bool PERM>
+// REG: an identity bin of a constant-length plan (row_len > 0): the row
+// bounds are computed, so the first wait of a tile is for its chunks.  A
+// separate instantiation because a load that may or may not have happened
+// before the chunk loads makes the compiler wait for every outstanding
+// access there, the previous tile's y store included: a second round trip
+// per tile.
+template <int L, typename T, typename IT, typename PT, typename QT, int CHUNKS, bool PERM, bool REG>
 __device__ __forceinline__ void spmv_vec_rows(const launch_bin& lb, const launch_table& tab,
                                               const PT* __restrict__ indptr, const IT* __restrict__ indices,
                                               const T* __restrict__ data, const QT* __restrict__ perm,
@@ -281,7 +287,14 @@ __device__ __forceinline__ void spmv_vec_rows(const launch_bin& lb, const launch
     const bool active = rr < lb.count;
     A acc = A(0);
     work_item w{0, 0, 0, 0};
-    if (active) w = load_item(tab, lb, indptr, rr);
+    if (REG) {
+      w.row = rr;
+      w.begin = tab.indptr_base + rr * tab.row_len;
+      w.end = w.begin + tab.row_len;
+      w.out = rr;
+    } else if (active) {
+      w = load_item(tab, lb, indptr, rr);
+    }
     // no lane's chunks reach past the arrays' end: whole-chunk loads only (a
     // wave-uniform choice).  A lane's chunks start below w.end, 4-aligned, and
     // a trip reaches 4 * L * (CHUNKS - 1) elements further.
@@ -344,8 +357,15 @@ __global__ __launch_bounds__(kLdsxThreads) void spmv_vec_kernel(
     typename spmv_acc_of<T>::type* __restrict__ partials, launch_table tab, int64_t x_len, int64_t nnz) {
   const T* __restrict__ x = stage_x<T, true>(x_in, x_len);
   const launch_bin lb = tab.b[find_bin(tab, blockIdx.x)];
-#define FSP_VEC_ROWS(LN) \
-  spmv_vec_rows<LN, T, IT, PT, QT, CHUNKS, PERM>(lb, tab, indptr, indices, data, perm, x, y, partials, nnz)
+  const bool reg = tab.row_len > 0 && !lb.is_segment && lb.rows == nullptr;  // uniform over the launch
+#define FSP_VEC_ROWS(LN)                                                                                     \
+  if (reg) {                                                                                               \
+    spmv_vec_rows<LN, T, IT, PT, QT, CHUNKS, PERM, true>(lb, tab, indptr, indices, data, perm, x, y, partials, \
+                                                         nnz);                                             \
+  } else {                                                                                                 \
+    spmv_vec_rows<LN, T, IT, PT, QT, CHUNKS, PERM, false>(lb, tab, indptr, indices, data, perm, x, y,         \
+                                                          partials, nnz);                                  \
+  }
   switch (lb.lanes_log2) {  // uniform over the workgroup
     case 0: FSP_VEC_ROWS(1); break;
     case 1: FSP_VEC_ROWS(2); break;
