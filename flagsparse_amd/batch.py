@@ -160,6 +160,9 @@ class CsrBatch:
         self.handle = CsrHandle(self.indices, self.indptr, (self.total_rows, self.total_cols))
         lengths = self.indptr[1:] - self.indptr[:-1] if self.total_rows else self.indptr[:0]
         self.max_row = int(lengths.max().item()) if self.total_rows else 0
+        # every row of every matrix the same length: the parts loop computes
+        # row bounds (the packed indptr starts at 0)
+        self.row_len = _plan._row_len_if_constant(self.total_rows, 0, self.nnz, self.max_row)
 
     # ------------------------------------------------------------- tuning
     def params(self, kind, n_dense=1, dtype=None, tune=True, data=None):
@@ -295,7 +298,8 @@ class CsrBatch:
             _lib.load().fsp_spmv_csr_parts(
                 _lib.value_code(data.dtype), _lib.index_code(self.indices.dtype), _lib.index_code(self.indptr.dtype),
                 self.indptr.data_ptr(), _lib.ptr(self.indices), _lib.ptr(data), _lib.ptr(x), y.data_ptr(),
-                int(data.numel()), int(parts.shape[0]), parts.data_ptr(), int(lanes), _lib.stream_ptr(self.device),
+                int(data.numel()), int(parts.shape[0]), parts.data_ptr(), int(lanes), self.row_len, 0,
+                _lib.stream_ptr(self.device),
             ),
             "fsp_spmv_csr_parts",
         )

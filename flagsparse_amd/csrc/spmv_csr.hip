@@ -384,12 +384,16 @@ __global__ __launch_bounds__(kLdsxThreads) void spmv_vec_kernel(
 // when the window fits kLdsxBytes, else gathers from global memory, and runs
 // the rows with the vector-chunk loop (L lanes per row).  A batch of
 // matrices maps each matrix's rows (cut into parts of bounded nonzero count)
-// to parts whose window is the matrix's column range.
-template <int L, typename T, typename IT, typename PT>
+// to parts whose window is the matrix's column range.  REG (row_len > 0):
+// every row holds row_len nonzeros from indptr_base + r * row_len, so the
+// bounds are computed and a pass's first wait is for its chunks (see
+// spmv_vec_rows).
+template <int L, typename T, typename IT, typename PT, bool REG>
 __device__ __forceinline__ void spmv_part_rows(const PT* __restrict__ indptr, const IT* __restrict__ indices,
                                                const T* __restrict__ data, const T* __restrict__ xb,
                                                int64_t xoff, T* __restrict__ y, int64_t row_begin,
-                                               int64_t row_end, int64_t nnz) {
+                                               int64_t row_end, int64_t nnz, int64_t row_len,
+                                               int64_t indptr_base) {
   using A = typename spmv_acc_of<T>::type;
   constexpr int64_t per_wave = 64 / L;
   constexpr int64_t per_pass = per_wave * kLdsxWaves;
@@ -401,7 +405,10 @@ __device__ __forceinline__ void spmv_part_rows(const PT* __restrict__ indptr, co
     const int64_t r = base + wave * per_wave + sub;
     const bool active = r < row_end;
     int64_t b = 0, e = 0;
-    if (active) {
+    if (REG) {
+      b = indptr_base + r * row_len;
+      e = b + row_len;
+    } else if (active) {
       b = static_cast<int64_t>(indptr[r]);
       e = static_cast<int64_t>(indptr[r + 1]);
     }
@@ -438,7 +445,7 @@ __global__ __launch_bounds__(kLdsxThreads) void spmv_parts_kernel(const PT* __re
                                                                   const T* __restrict__ data,
                                                                   const T* __restrict__ x, T* __restrict__ y,
                                                                   const int64_t* __restrict__ parts, int lanes_log2,
-                                                                  int64_t nnz) {
+                                                                  int64_t nnz, int64_t row_len, int64_t indptr_base) {
   extern __shared__ __align__(16) unsigned char fsp_part_mem[];
   T* xs = reinterpret_cast<T*>(fsp_part_mem);
   const int64_t* pt = parts + 4 * static_cast<int64_t>(blockIdx.x);
@@ -449,10 +456,13 @@ __global__ __launch_bounds__(kLdsxThreads) void spmv_parts_kernel(const PT* __re
   }
   __syncthreads();
 #define FSP_PART_ROWS(LN)                                                                                   \
-  if (fits) {                                                                                             \
-    spmv_part_rows<LN, T, IT, PT>(indptr, indices, data, xs, lo, y, row_begin, row_end, nnz);            \
+  if (fits && row_len > 0) {                                                                              \
+    spmv_part_rows<LN, T, IT, PT, true>(indptr, indices, data, xs, lo, y, row_begin, row_end, nnz, row_len,  \
+                                        indptr_base);                                                     \
+  } else if (fits) {                                                                                      \
+    spmv_part_rows<LN, T, IT, PT, false>(indptr, indices, data, xs, lo, y, row_begin, row_end, nnz, 0, 0);   \
   } else {                                                                                                \
-    spmv_part_rows<LN, T, IT, PT>(indptr, indices, data, x, 0, y, row_begin, row_end, nnz);              \
+    spmv_part_rows<LN, T, IT, PT, false>(indptr, indices, data, x, 0, y, row_begin, row_end, nnz, 0, 0);     \
   }
   switch (lanes_log2) {  // uniform
     case 2: FSP_PART_ROWS(4); break;
@@ -636,9 +646,14 @@ extern "C" int fsp_spmv_csr(int32_t value_type, int32_t index_type, int32_t indp
 extern "C" int fsp_spmv_csr_parts(int32_t value_type, int32_t index_type, int32_t indptr_type,
                                   const void* indptr_dev, const void* indices_dev, const void* data_dev,
                                   const void* x_dev, void* y_dev, int64_t nnz, int64_t n_parts,
-                                  const int64_t* parts_dev, int32_t lanes, void* stream) {
+                                  const int64_t* parts_dev, int32_t lanes, int64_t row_len,
+                                  int64_t indptr_base, void* stream) {
   if (nnz < 0 || n_parts < 0 || (lanes != 4 && lanes != 8 && lanes != 16)) {
     return fail(FSP_ERR_ARG, "fsp_spmv_csr_parts: bad arguments (lanes must be 4, 8 or 16)");
+  }
+  if (row_len < 0 || indptr_base < 0) {
+    return fail(FSP_ERR_ARG, "fsp_spmv_csr_parts: row_len %lld / indptr_base %lld negative", (long long)row_len,
+                (long long)indptr_base);
   }
   if (n_parts == 0) return FSP_OK;
   if (n_parts > 0x7fffffff || parts_dev == nullptr || indptr_dev == nullptr || y_dev == nullptr ||
@@ -657,7 +672,8 @@ extern "C" int fsp_spmv_csr_parts(int32_t value_type, int32_t index_type, int32_
         hipLaunchKernelGGL((spmv_parts_kernel<T, IT, PT>), dim3(static_cast<unsigned>(n_parts)), dim3(kLdsxThreads),
                            static_cast<size_t>(kLdsxBytes), s, static_cast<const PT*>(indptr_dev),
                            static_cast<const IT*>(indices_dev), static_cast<const T*>(data_dev),
-                           static_cast<const T*>(x_dev), static_cast<T*>(y_dev), parts_dev, ll, nnz);
+                           static_cast<const T*>(x_dev), static_cast<T*>(y_dev), parts_dev, ll, nnz, row_len,
+                           indptr_base);
         return check_launch("spmv_parts");
       });
     });

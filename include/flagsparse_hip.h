@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSP_ABI_VERSION 13
+#define FSP_ABI_VERSION 14
 #define FSP_MAX_BINS 8
 
 typedef enum fsp_value_type {
@@ -179,11 +179,15 @@ int fsp_spmv_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
  * The table is read on the device and not checked: every row_end must be a
  * valid row of indptr_dev (indptr has row_end + 1 entries), col_lo <= col_hi
  * must lie within x, and the rows' column ids within their part's window.
- * The Python layer builds it from a validated batch (CsrBatch.parts). */
+ * The Python layer builds it from a validated batch (CsrBatch.parts).
+ * row_len > 0 states that every row holds exactly row_len nonzeros and
+ * indptr[i] = indptr_base + i * row_len (the caller vouches for it): the
+ * kernel then computes row bounds instead of loading them; 0 loads them. */
 int fsp_spmv_csr_parts(int32_t value_type, int32_t index_type, int32_t indptr_type,
                        const void* indptr_dev, const void* indices_dev, const void* data_dev,
                        const void* x_dev, void* y_dev, int64_t nnz, int64_t n_parts,
-                       const int64_t* parts_dev, int32_t lanes, void* stream);
+                       const int64_t* parts_dev, int32_t lanes, int64_t row_len,
+                       int64_t indptr_base, void* stream);
 
 /* ---- SpMM ----------------------------------------------------------------
  * C[n_rows, n_dense] = A B with B row-major (ldb >= n_dense) and C row-major

@@ -373,12 +373,15 @@ def test_spmv_parts_entry_validation(device):
     y = torch.zeros(4, device=device)
     vals = torch.zeros(9, device=device)
 
-    def call(data_ptr, n_parts, lanes):
+    def call(data_ptr, n_parts, lanes, row_len=0):
         return lib.fsp_spmv_csr_parts(0, 0, 0, one.data_ptr(), one.data_ptr(), data_ptr, y.data_ptr(),
-                                      y.data_ptr(), 4, n_parts, parts.data_ptr(), lanes, _lib.stream_ptr(device))
+                                      y.data_ptr(), 4, n_parts, parts.data_ptr(), lanes, row_len, 0,
+                                      _lib.stream_ptr(device))
 
     with pytest.raises(RuntimeError, match="lanes"):
         _lib.check(call(y.data_ptr(), 1, 5), "fsp_spmv_csr_parts")
+    with pytest.raises(RuntimeError, match="row_len"):
+        _lib.check(call(y.data_ptr(), 1, 4, row_len=-1), "fsp_spmv_csr_parts")
     with pytest.raises(RuntimeError, match="aligned"):
         _lib.check(call(vals[1:].data_ptr(), 1, 4), "fsp_spmv_csr_parts")
     assert call(y.data_ptr(), 0, 4) == 0  # no parts: nothing to launch
@@ -401,3 +404,35 @@ def test_per_matrix_loops_take_strided_operands(device):
         b.spmv(xs.contiguous(), out=y_strided)
     with pytest.raises(ValueError, match="contiguous"):
         b.spmm(Bs.contiguous(), out=torch.empty(b.total_rows, 13, device=device))
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+def test_constant_row_length_batch_computes_bounds(device, dtype, lanes):
+    """A batch whose matrices all have 16 nonzeros per row (two replicas of
+    one uniform matrix and a uniform matrix of another shape): the parts
+    loop computes the row bounds; bitwise equal to the loaded bounds and to
+    the per-matrix launches, and within tolerance of the oracle."""
+    from flagsparse_amd.autotune import TuneParams
+
+    mats, golds = [], []
+    for k, (m, n) in enumerate(((300, 2048), (300, 2048), (1000, 4096))):
+        ip, ix, v, _ = S.uniform_rows(m, n, 16, seed=k % 2)
+        d, i, p = S.to_torch(ip, ix, v, device, dtype, torch.int32)
+        mats.append((d, i, p, (m, n)))
+        golds.append((ip, ix, v, (m, n)))
+    b = CsrBatch.from_matrices(mats)
+    assert b.row_len == 16
+    x = torch.rand(b.total_cols, device=device, dtype=dtype)
+    params = TuneParams(unroll=4, lanes_n=lanes, variant=5, part_nnz=1024)
+    y = b.spmv(x, params=params)
+    assert torch.equal(y, b.spmv_per_matrix(x, params=params))
+    b.row_len = 0
+    assert torch.equal(y, b.spmv(x, params=params))
+    xh = x.double().cpu().numpy()
+    r0 = c0 = 0
+    for ip, ix, v, (m, n) in golds:
+        vr = torch.from_numpy(v).to(dtype).double().numpy()
+        ref = O.spmv(ip, ix, vr, xh[c0:c0 + n], (m, n))
+        assert rel_err(y[r0:r0 + m], ref) <= REL_TOL[dtype]
+        r0, c0 = r0 + m, c0 + n
