@@ -350,6 +350,80 @@ __device__ __forceinline__ void spmv_vec_rows(const launch_bin& lb, const launch
   }
 }
 
+// Identity bin of a constant-length plan whose rows each fit one trip of
+// the lane group (indptr_base and row_len multiples of 4, row_len <=
+// 4 * L * CHUNKS, every row inside the arrays): software-pipelined over row
+// tiles.  The next tile's chunks load before this tile's are summed, from
+// addresses that need no load, so a wave keeps one tile in flight while it
+// works on another.  The loads are unconditional (a finished or short lane
+// reloads a chunk it already holds, and masks it at the FMA): a load under a
+// branch would make the compiler wait for everything outstanding at the join.
+// Per lane the terms are added in the order of spmv_vec_rows (bitwise equal).
+template <int L, typename T, typename IT, int CHUNKS>
+__device__ __forceinline__ void spmv_vec_rows_pipe(const launch_bin& lb, const launch_table& tab,
+                                                   const IT* __restrict__ indices, const T* __restrict__ data,
+                                                   const T* __restrict__ x, T* __restrict__ y) {
+  using A = typename spmv_acc_of<T>::type;
+  constexpr int64_t per_wave = 64 / L;
+  constexpr int64_t per_block = per_wave * kLdsxWaves;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int sub = lane / L;
+  const int l = lane % L;
+  const int64_t len = tab.row_len;
+  const int64_t last = lb.count - 1;
+  const tile_range tr = bin_tiles(lb, per_block);
+  int64_t off[CHUNKS];  // this lane's chunk offsets inside a row (clamped to 0 past its end)
+#pragma unroll
+  for (int u = 0; u < CHUNKS; ++u) {
+    const int64_t o = 4 * l + 4 * L * u;
+    off[u] = o < len ? o : 0;
+  }
+  auto row_base = [&](int64_t tile) {
+    const int64_t rr = tile * per_block + wave * per_wave + sub;
+    return tab.indptr_base + (rr < last ? rr : last) * len;
+  };
+  chunk4<IT> c[CHUNKS];
+  chunk4<T> v[CHUNKS];
+  if (tr.first < tr.end) {
+    const int64_t b = row_base(tr.first);
+#pragma unroll
+    for (int u = 0; u < CHUNKS; ++u) {
+      c[u] = load_chunk_nt(indices + b + off[u]);
+      v[u] = load_chunk_nt(data + b + off[u]);
+    }
+  }
+  for (int64_t tile = tr.first; tile < tr.end; tile += tr.step) {
+    const int64_t next = tile + tr.step < tr.end ? tile + tr.step : tile;
+    const int64_t bn = row_base(next);
+    chunk4<IT> cn[CHUNKS];
+    chunk4<T> vn[CHUNKS];
+#pragma unroll
+    for (int u = 0; u < CHUNKS; ++u) {
+      cn[u] = load_chunk_nt(indices + bn + off[u]);
+      vn[u] = load_chunk_nt(data + bn + off[u]);
+    }
+    const int64_t rr = tile * per_block + wave * per_wave + sub;
+    A acc = A(0);
+#pragma unroll
+    for (int u = 0; u < CHUNKS; ++u) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const bool ok = 4 * l + 4 * L * u + j < len;
+        const A t = fmadd(static_cast<A>(v[u].v[j]), static_cast<A>(x[static_cast<int64_t>(c[u].v[j])]), acc);
+        acc = ok ? t : acc;
+      }
+    }
+    acc = group_sum(acc, L);
+    if (rr <= last && l == 0) y[rr] = static_cast<T>(acc);
+#pragma unroll
+    for (int u = 0; u < CHUNKS; ++u) {
+      c[u] = cn[u];
+      v[u] = vn[u];
+    }
+  }
+}
+
 template <typename T, typename IT, typename PT, typename QT, int CHUNKS, bool PERM>
 __global__ __launch_bounds__(kLdsxThreads) void spmv_vec_kernel(
     const PT* __restrict__ indptr, const IT* __restrict__ indices, const T* __restrict__ data,
@@ -358,8 +432,13 @@ __global__ __launch_bounds__(kLdsxThreads) void spmv_vec_kernel(
   const T* __restrict__ x = stage_x<T, true>(x_in, x_len);
   const launch_bin lb = tab.b[find_bin(tab, blockIdx.x)];
   const bool reg = tab.row_len > 0 && !lb.is_segment && lb.rows == nullptr;  // uniform over the launch
+  // one trip per row, every row inside the arrays: the pipelined loop
+  const bool pipe = !PERM && reg && (tab.indptr_base & 3) == 0 && (tab.row_len & 3) == 0 &&
+                    tab.indptr_base + lb.count * tab.row_len <= nnz;
 #define FSP_VEC_ROWS(LN)                                                                                     \
-  if (reg) {                                                                                               \
+  if (pipe && tab.row_len <= 4 * (LN) * CHUNKS) {                                                          \
+    spmv_vec_rows_pipe<LN, T, IT, CHUNKS>(lb, tab, indices, data, x, y);                                    \
+  } else if (reg) {                                                                                        \
     spmv_vec_rows<LN, T, IT, PT, QT, CHUNKS, PERM, true>(lb, tab, indptr, indices, data, perm, x, y, partials, \
                                                          nnz);                                             \
   } else {                                                                                                 \

@@ -384,20 +384,23 @@ def test_spmv_vector_chunk_variant_rejects_misaligned_and_unroll(device):
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.bfloat16], ids=["f32", "f64", "bf16"])
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
-@pytest.mark.parametrize("base", [0, 40])
-def test_spmv_constant_row_length_computes_bounds(device, dtype, variant, base):
-    """Rows that all hold 16 nonzeros: the plan keeps row_len = 16 and the
+@pytest.mark.parametrize("per_row,base", [(16, 0), (16, 40), (24, 0), (13, 0), (16, 2)])
+def test_spmv_constant_row_length_computes_bounds(device, dtype, variant, per_row, base):
+    """Rows that all hold per_row nonzeros: the plan keeps row_len and the
     kernels compute each row's bounds instead of loading them.  Bitwise equal
     to the same plan with row_len cleared (bounds loaded from indptr), on
-    every loop variant, also for an indptr that starts at 40 (the index and
-    value arrays then begin 40 elements in)."""
+    every loop variant, also for an indptr that starts at 40 or 2 (the index
+    and value arrays then begin that many elements in).  Variant 4 takes its
+    software-pipelined loop for 16 and 24 per row at bases 0 and 40 (24: two
+    of a row's eight lanes hold no chunk), and its plain computed-bounds loop
+    for 13 per row and base 2."""
     import dataclasses
 
     from flagsparse_amd import _plan
     from flagsparse_amd._matrix import CsrHandle
 
     n_rows, n_cols = 5003, 4096
-    ip, ix, v, x = synthetic.uniform_rows(n_rows, n_cols, 16, seed=base + variant)
+    ip, ix, v, x = synthetic.uniform_rows(n_rows, n_cols, per_row, seed=base + variant)
     rng = np.random.default_rng(base)
     ip_b = ip + base
     ix_b = np.concatenate([rng.integers(0, n_cols, base), ix])
@@ -407,7 +410,7 @@ def test_spmv_constant_row_length_computes_bounds(device, dtype, variant, base):
     h = CsrHandle(idx, ptr, (n_rows, n_cols))
     cfg = _plan.spmv_config(unroll=4)
     plan = h.plan(cfg)
-    assert plan.identity and plan.row_len == 16 and plan.indptr_base == base
+    assert plan.identity and plan.row_len == per_row and plan.indptr_base == base
     got = h.spmv(data, xt, config=cfg, variant=variant)
     h._plans[cfg] = dataclasses.replace(plan, row_len=0, indptr_base=0)
     loaded = h.spmv(data, xt, config=cfg, variant=variant)
