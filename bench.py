@@ -442,10 +442,14 @@ def bench_spmv_large(dev, steps, warmup):
     gbs = nbytes / (med * 1e-3) / 1e9
     probe = bench_read_probe(dev, idx, data, steps, warmup)
     probe["spmv_frac_of_best"] = round(gbs / probe["best_gbs"], 4)
+    # rows of one length: the loop computes row bounds and reads no row pointer,
+    # so the bytes it moves are the CSR bytes less indptr (PMC-measured below)
+    computed = handle.plan(p.spmv_config()).row_len > 0
+    traffic = measured_traffic(SPMV_TRAFFIC_SUMMARY) if p.variant == 4 and computed else None
     del ptr, idx, data, x, y, handle
     return {"rows": m, "cols": n, "nnz": nnz, "bytes": nbytes, "event_ms_median": round(med, 5),
-            "tuned": {k: v for k, v in p.__dict__.items()},
-            "traffic": measured_traffic(SPMV_TRAFFIC_SUMMARY) if p.variant == 4 else None,
+            "tuned": {k: v for k, v in p.__dict__.items()}, "row_bounds": "computed" if computed else "loaded",
+            "traffic": traffic, "moved_gbs": round(traffic / (med * 1e-3) / 1e9, 2) if traffic else None,
             "gbs": round(gbs, 2), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "read_probe": probe}
 
 

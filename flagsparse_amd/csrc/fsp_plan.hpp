@@ -43,7 +43,7 @@ struct launch_table {
   const int32_t* seg_row;
   const int64_t* seg_begin;
   const int64_t* seg_end;
-  int64_t row_len;      // > 0: rows of exactly this length (bounds computed, not loaded)
+  int64_t row_len;      // > 0: rows of exactly this length (the SpMV vector-chunk loop computes bounds)
   int64_t indptr_base;  // indptr[0] when row_len > 0
 };
 
@@ -97,13 +97,8 @@ __device__ __forceinline__ work_item load_item(const launch_table& t, const laun
     w.out = rr;
   } else {
     w.row = lb.rows ? static_cast<int64_t>(lb.rows[rr]) : rr;
-    if (t.row_len > 0) {  // uniform over the launch: no dependent row-pointer load
-      w.begin = t.indptr_base + w.row * t.row_len;
-      w.end = w.begin + t.row_len;
-    } else {
-      w.begin = static_cast<int64_t>(indptr[w.row]);
-      w.end = static_cast<int64_t>(indptr[w.row + 1]);
-    }
+    w.begin = static_cast<int64_t>(indptr[w.row]);
+    w.end = static_cast<int64_t>(indptr[w.row + 1]);
     w.out = w.row;
   }
   return w;

@@ -424,23 +424,29 @@ __device__ __forceinline__ void spmv_vec_rows_pipe(const launch_bin& lb, const l
   }
 }
 
-template <typename T, typename IT, typename PT, typename QT, int CHUNKS, bool PERM>
+// REG: a launch whose bins are all identity bins of a constant-length plan
+// (tab.row_len > 0, no segments, no permutation; the host checks).  Its own
+// kernel, so the general one keeps its register budget: the computed-bounds
+// paths raise the scalar registers past what two 16-wave workgroups per CU
+// allow.
+template <typename T, typename IT, typename PT, typename QT, int CHUNKS, bool PERM, bool REG>
 __global__ __launch_bounds__(kLdsxThreads) void spmv_vec_kernel(
     const PT* __restrict__ indptr, const IT* __restrict__ indices, const T* __restrict__ data,
     const QT* __restrict__ perm, const T* __restrict__ x_in, T* __restrict__ y,
     typename spmv_acc_of<T>::type* __restrict__ partials, launch_table tab, int64_t x_len, int64_t nnz) {
   const T* __restrict__ x = stage_x<T, true>(x_in, x_len);
   const launch_bin lb = tab.b[find_bin(tab, blockIdx.x)];
-  const bool reg = tab.row_len > 0 && !lb.is_segment && lb.rows == nullptr;  // uniform over the launch
   // one trip per row, every row inside the arrays: the pipelined loop
-  const bool pipe = !PERM && reg && (tab.indptr_base & 3) == 0 && (tab.row_len & 3) == 0 &&
-                    tab.indptr_base + lb.count * tab.row_len <= nnz;
+  const bool pipe = REG && (tab.indptr_base & 3) == 0 && (tab.row_len & 3) == 0 &&
+                    tab.indptr_base + lb.count * tab.row_len <= nnz;  // uniform over the launch
 #define FSP_VEC_ROWS(LN)                                                                                     \
-  if (pipe && tab.row_len <= 4 * (LN) * CHUNKS) {                                                          \
-    spmv_vec_rows_pipe<LN, T, IT, CHUNKS>(lb, tab, indices, data, x, y);                                    \
-  } else if (reg) {                                                                                        \
-    spmv_vec_rows<LN, T, IT, PT, QT, CHUNKS, PERM, true>(lb, tab, indptr, indices, data, perm, x, y, partials, \
-                                                         nnz);                                             \
+  if constexpr (REG) {                                                                                     \
+    if (pipe && tab.row_len <= 4 * (LN) * CHUNKS) {                                                        \
+      spmv_vec_rows_pipe<LN, T, IT, CHUNKS>(lb, tab, indices, data, x, y);                                  \
+    } else {                                                                                               \
+      spmv_vec_rows<LN, T, IT, PT, QT, CHUNKS, PERM, true>(lb, tab, indptr, indices, data, perm, x, y,        \
+                                                           partials, nnz);                                 \
+    }                                                                                                      \
   } else {                                                                                                 \
     spmv_vec_rows<LN, T, IT, PT, QT, CHUNKS, PERM, false>(lb, tab, indptr, indices, data, perm, x, y,         \
                                                           partials, nnz);                                  \
@@ -643,17 +649,31 @@ static int launch_spmv(const void* indptr, const void* indices, const void* data
     case 3: FSP_SPMV_KERNEL(spmv_bins_kernel, U, kLdsxWaves, true); break;   \
     default: FSP_SPMV_KERNEL(spmv_bins_kernel, U, FSP_WAVES, false); break;  \
   }
-#define FSP_SPMV_VEC(CH)                                                                               \
-  hipLaunchKernelGGL((spmv_vec_kernel<T, IT, PT, QT, CH, PERM>), dim3(blocks), dim3(kLdsxThreads),        \
+#define FSP_SPMV_VEC(CH, REG)                                                                          \
+  hipLaunchKernelGGL((spmv_vec_kernel<T, IT, PT, QT, CH, PERM, REG>), dim3(blocks), dim3(kLdsxThreads),   \
                      static_cast<size_t>(x_bytes), s, static_cast<const PT*>(indptr),                    \
                      static_cast<const IT*>(indices), static_cast<const T*>(data),                        \
                      static_cast<const QT*>(perm), static_cast<const T*>(x), static_cast<T*>(y), partials, \
                      tab, x_len, nnz)
     if (vec) {
-      if (unroll == 4) {
-        FSP_SPMV_VEC(1);
-      } else {
-        FSP_SPMV_VEC(2);
+      // constant-length plan, identity bins only, no segments: row bounds computed
+      bool reg = !PERM && plan->row_len > 0 && plan->n_segments == 0;
+      for (int i = 0; i < plan->n_bins; ++i) reg = reg && (plan->bins[i].count == 0 || plan->bins[i].rows == nullptr);
+      if constexpr (!PERM) {
+        if (reg) {
+          if (unroll == 4) {
+            FSP_SPMV_VEC(1, true);
+          } else {
+            FSP_SPMV_VEC(2, true);
+          }
+        }
+      }
+      if (!reg) {
+        if (unroll == 4) {
+          FSP_SPMV_VEC(1, false);
+        } else {
+          FSP_SPMV_VEC(2, false);
+        }
       }
     } else if (unroll <= 2) {
       FSP_SPMV_LAUNCH(2);

@@ -386,11 +386,12 @@ def test_spmv_vector_chunk_variant_rejects_misaligned_and_unroll(device):
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("per_row,base", [(16, 0), (16, 40), (24, 0), (13, 0), (16, 2)])
 def test_spmv_constant_row_length_computes_bounds(device, dtype, variant, per_row, base):
-    """Rows that all hold per_row nonzeros: the plan keeps row_len and the
-    kernels compute each row's bounds instead of loading them.  Bitwise equal
-    to the same plan with row_len cleared (bounds loaded from indptr), on
-    every loop variant, also for an indptr that starts at 40 or 2 (the index
-    and value arrays then begin that many elements in).  Variant 4 takes its
+    """Rows that all hold per_row nonzeros: the plan keeps row_len, and the
+    vector-chunk loop (variant 4) computes each row's bounds instead of
+    loading them; variants 0-3 load them.  Bitwise equal to the same plan
+    with row_len cleared (bounds loaded from indptr), on every loop variant,
+    also for an indptr that starts at 40 or 2 (the index and value arrays
+    then begin that many elements in).  Variant 4 takes its
     software-pipelined loop for 16 and 24 per row at bases 0 and 40 (24: two
     of a row's eight lanes hold no chunk), and its plain computed-bounds loop
     for 13 per row and base 2."""
