@@ -13,8 +13,10 @@
 //   2  spmv_simple_kernel and 3 spmv_bins_kernel with x staged in LDS by
 //      1024-thread workgroups;
 //   4  spmv_vec_kernel: the index / value streams read as 4-element chunks,
-//      x in LDS.
-// Variants 0-3 are bitwise equal.  spmv_parts_kernel (fsp_spmv_csr_parts)
+//      x in LDS; a plan whose rows all have one length (plan.row_len) runs
+//      its own instantiation that computes the row bounds and, when a row
+//      fits one trip, software-pipelines the row tiles.
+// Variants 0-3 are bitwise equal, and so are variant 4's two forms.  spmv_parts_kernel (fsp_spmv_csr_parts)
 // runs row-range parts, each with its own column window of x in LDS (the
 // batched SpMV).
 #include <atomic>
