@@ -16,7 +16,7 @@ from ._build import LIB_PATH
 FSP_F32, FSP_F64, FSP_F16, FSP_BF16, FSP_F8E4M3 = 0, 1, 2, 3, 4
 FSP_I32, FSP_I64 = 0, 1
 FSP_MAX_BINS = 8
-ABI_VERSION = 14
+ABI_VERSION = 15
 NOALLOC_SCRATCH_BYTES = 16 << 20
 
 _VALUE_CODES = {
@@ -96,6 +96,11 @@ SIGNATURES = {
         _c_i32,
         [_c_i32, _c_i32, _c_i32, _c_vp, _c_vp, _c_vp, _c_i32, _c_vp, _c_vp, _c_i64, _c_vp, _c_i64,
          _c_i64, _c_i64, _c_i64, _c_i64, _c_i32, ctypes.POINTER(FspCsrPlan), _c_vp],
+    ),
+    "fsp_spmm_csr_trans_sddmm": (
+        _c_i32,
+        [_c_i32, _c_i32, _c_i32, _c_vp, _c_vp, _c_vp, _c_i32, _c_vp, _c_vp, _c_i64, _c_vp, _c_i64, _c_vp, _c_i64,
+         _c_vp, _c_i64, _c_i64, _c_i64, _c_i64, _c_i32, ctypes.POINTER(FspCsrPlan), _c_vp],
     ),
     "fsp_spmm_csr_noalloc": (
         _c_i32,

@@ -116,6 +116,40 @@ class CsrHandle:
         )
         return C
 
+    def spmm_trans_sddmm(self, data, dC, B, dB=None, dvalues=None):
+        """The backward of C = A B in one pass over the CSC:
+        dB = A^T dC and dvalues[p] = <dC[row p], B[col p]> (CSR order), from
+        one gather of dC's rows (fsp_spmm_csr_trans_sddmm).  dB is bitwise
+        equal to spmm(data, dC, transpose=True).  Returns None when n_dense
+        does not fit one lane group of the kernel (the caller then runs the
+        transposed SpMM and the SDDMM)."""
+        n_dense = int(dC.shape[1])
+        elem = data.element_size()
+        device = dC.device
+        dB = dB if dB is not None else torch.empty((self.n_cols, n_dense), dtype=data.dtype, device=device)
+        dvalues = dvalues if dvalues is not None else torch.empty(self.nnz, dtype=data.dtype, device=device)
+        vec = 16 // elem
+        wide = all(t.data_ptr() % 16 == 0 and int(t.stride(0)) % vec == 0 for t in (dC, B, dB)) and n_dense % vec == 0
+        if n_dense > 64 * (vec if wide else 1):
+            return None
+        G = _plan.lanes_for_dense(n_dense, elem)
+        cfg = _plan.spmm_config(G)
+        colptr, rowind, perm = self.csc()
+        plan = self.csc_plan(cfg)
+        data_t = gather_values(data, perm) if self.nnz else data
+        partials = plan.partials(n_dense, _acc_dtype(data.dtype), device)
+        st = plan.struct(partials)
+        _lib.check(
+            _lib.load().fsp_spmm_csr_trans_sddmm(
+                _lib.value_code(data.dtype), _lib.index_code(rowind.dtype), _lib.index_code(colptr.dtype),
+                _lib.ptr(colptr), _lib.ptr(rowind), _lib.ptr(data_t), _lib.index_code(perm.dtype), _lib.ptr(perm),
+                dC.data_ptr(), int(dC.stride(0)), B.data_ptr(), int(B.stride(0)), dB.data_ptr(), int(dB.stride(0)),
+                dvalues.data_ptr(), self.n_cols, n_dense, self.nnz, self.n_rows, G, st, _lib.stream_ptr(device),
+            ),
+            "fsp_spmm_csr_trans_sddmm",
+        )
+        return dB, dvalues
+
     def spmv(self, data, x, transpose=False, out=None, config=None, max_blocks=0, xcd_groups=0, variant=0,
              values_via_perm=True):
         """y = A x or A^T x.  transpose with values_via_perm=False gathers the

@@ -3,9 +3,15 @@
 C = A B with A given by (values, indices, indptr, shape).  Backward:
 
 * dB = A^T dC: the transposed SpMM kernel over the handle's cached stable CSC
-  permutation, values read as values[perm[q]] -- no transposed copy of A, no
-  atomics;
+  permutation (values gathered into CSC order per call; no transposed copy
+  of A's indices, no atomics);
 * dA.values = SDDMM(dC, B) on A's pattern: dvalues[p] = <dC[row p], B[col p]>.
+
+When both are needed they come from one kernel pass over the CSC
+(``CsrHandle.spmm_trans_sddmm``): a slot that gathers dC[i] for dB[j] also
+forms <dC[i], B[j]>, so dC's rows are gathered once instead of once per
+gradient.  Operands too wide for one lane group of that kernel take the two
+kernels.
 
 The reference has no autograd (no backward anywhere in its sources); its
 forward is flagsparse_spmm_csr (spmm_csr.py:3981) and the gradient pieces map
@@ -80,6 +86,11 @@ class _SpmmCsrFunction(torch.autograd.Function):
         if dC.dtype != values.dtype:
             dC = dC.to(values.dtype)
         d_values = d_B = None
+        if ctx.needs_input_grad[0] and ctx.needs_input_grad[1]:
+            fused = matrix.handle.spmm_trans_sddmm(values, dC, B)
+            if fused is not None:
+                d_B, d_values = fused
+                return d_values, d_B, None
         if ctx.needs_input_grad[0]:
             d_values = matrix.handle.sddmm(dC, B, alpha=1.0, beta=0.0)
         if ctx.needs_input_grad[1]:

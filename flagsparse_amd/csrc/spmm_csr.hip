@@ -12,6 +12,12 @@
 // Over a CSC with perm != NULL the same kernel computes A^T B, reading the
 // values as data[perm[q]] (no transposed copy of A).
 //
+// SDD (the differentiable SpMM's backward, over A's CSC): while a slot
+// accumulates dB[j] += a_q dC[i] it also has the gathered row dC[i] in
+// registers, so the same pass writes dvalues at the nonzero's CSR position,
+// dvalues[pos[q]] = <dC[i], B[j]> (B the forward's dense operand, its row j
+// loaded once per item).  One gather of dC rows serves both gradients.
+//
 // The loop is software-pipelined: while the B-row loads of one trip are in
 // flight a lane already loads the column ids / values of its next trip -- of
 // the same row, or the first trip of its next row -- so the index stream's
@@ -26,11 +32,55 @@
 
 namespace fsp {
 
-template <typename T, typename IT, typename PT, typename QT, int VEC, int UNROLL, bool PERM, bool OFF32>
+// <x, y> over a lane's VEC columns in the accumulation type.  16-bit values
+// go through v_dot2_f32_{bf16,f16}: two products and the running sum per
+// instruction, where converting each element and an FMA would take four.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+template <typename A, typename T, int VEC>
+__device__ __forceinline__ A row_dot(const vec_t<T, VEC>& x, const vec_t<T, VEC>& y) {
+  if constexpr (std::is_same<T, __bf16>::value && VEC % 2 == 0) {
+    float t = 0.0f;
+#pragma unroll
+    for (int v = 0; v < VEC; v += 2) {
+      const bf16x2_t a = {x.v[v], x.v[v + 1]};
+      const bf16x2_t c = {y.v[v], y.v[v + 1]};
+      t = __builtin_amdgcn_fdot2_f32_bf16(a, c, t, false);
+    }
+    return t;
+  } else if constexpr (std::is_same<T, _Float16>::value && VEC % 2 == 0) {
+    float t = 0.0f;
+#pragma unroll
+    for (int v = 0; v < VEC; v += 2) {
+      const f16x2_t a = {x.v[v], x.v[v + 1]};
+      const f16x2_t c = {y.v[v], y.v[v + 1]};
+      t = __builtin_amdgcn_fdot2(a, c, t, false);
+    }
+    return t;
+  } else {
+    A t = A(0);
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) t += static_cast<A>(x.v[v]) * static_cast<A>(y.v[v]);
+    return t;
+  }
+}
+
+// The backward's extra operands (SDD): the forward's B (row j of the item),
+// each nonzero's CSR position, and the dvalues output in CSR order.
+template <typename T, typename QT>
+struct sdd_args {
+  const T* Bj;
+  int64_t ldbj;
+  const QT* pos;
+  T* dvals;
+};
+
+template <typename T, typename IT, typename PT, typename QT, int VEC, int UNROLL, bool PERM, bool OFF32, bool SDD>
 __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
     const PT* __restrict__ indptr, const IT* __restrict__ indices, const T* __restrict__ data,
     const QT* __restrict__ perm, const T* __restrict__ B, int64_t ldb, T* __restrict__ C, int64_t ldc,
-    int64_t n_dense, int g_log2, typename acc_of<T>::type* __restrict__ partials, launch_table tab) {
+    int64_t n_dense, int g_log2, typename acc_of<T>::type* __restrict__ partials, launch_table tab,
+    sdd_args<T, QT> sd) {
   using A = typename acc_of<T>::type;
   using O = typename addr<OFF32>::off_t;  // element / byte offsets
   using P = typename std::conditional<OFF32, int32_t, int64_t>::type;  // nonzero positions
@@ -61,23 +111,29 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
   const O col_ld = col_ok ? col0 : O(0);  // lanes past n_dense read column 0
   const int64_t last_item = lb.count - 1;  // bins in the table are non-empty
 
-  // Item (row or segment) of this lane: output slot, nonzero range.
-  auto item = [&](int64_t r, O& out, P& beg, P& end) {
-    const bool live = r < lb.count && col_ok;
+  // Item (row or segment) of this lane: output slot, nonzero range, and the
+  // row it belongs to (its B row under SDD).  Under SDD the lanes past
+  // n_dense run their slot's nonzeros too (reading column 0, contributing 0
+  // to the dots): a dot's lane-group sum needs every lane of the group.
+  auto item = [&](int64_t r, O& out, P& beg, P& end, O& jrow) {
+    const bool live = r < lb.count && (SDD || col_ok);
     const int64_t rc = r < lb.count ? r : last_item;
-    O o;
+    O o, jr;
     P b, e;
     if (lb.is_segment) {
       o = static_cast<O>(rc);
+      jr = SDD ? static_cast<O>(tab.seg_row[rc]) : O(0);
       b = static_cast<P>(tab.seg_begin[rc]);
       e = static_cast<P>(tab.seg_end[rc]);
     } else {
       const int64_t row = lb.rows ? static_cast<int64_t>(lb.rows[rc]) : rc;
       o = static_cast<O>(row);
+      jr = o;
       b = static_cast<P>(indptr[row]);
       e = static_cast<P>(indptr[row + 1]);
     }
     out = o;
+    jrow = jr;
     beg = live ? b : P(0);
     end = live ? e : P(0);
   };
@@ -86,7 +142,7 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
   // stay raw: masking or converting them here would make the compiler wait
   // for every load in flight (the B rows included) right after the prefetch,
   // so the mask and the conversion are applied where they are consumed.
-  auto trip = [&](P p, P end, IT (&c)[UNROLL], T (&a)[UNROLL]) {
+  auto trip = [&](P p, P end, IT (&c)[UNROLL], T (&a)[UNROLL], QT (&pq)[UNROLL]) {
     const P last = end > 0 ? end - 1 : P(0);
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
@@ -95,25 +151,38 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
       c[u] = elem<OFF32>(indices, qc);
       const O qv = PERM ? static_cast<O>(elem<OFF32>(perm, qc)) : qc;
       a[u] = elem<OFF32>(data, qv);
+      if constexpr (SDD) pq[u] = elem<OFF32>(sd.pos, qc);
     }
+  };
+  // Row j of the forward's B for this lane's columns (zero past n_dense).
+  auto b_row = [&](O j) {
+    vec_t<T, VEC> r;
+    if constexpr (SDD) {
+      r = *addr<OFF32>::at(reinterpret_cast<const vec_t<T, VEC>*>(sd.Bj),
+                           (j * static_cast<O>(sd.ldbj) + col_ld) * static_cast<O>(sizeof(T)));
+    }
+    return r;
   };
 
   int64_t rr = tr.first * per_block + wave * per_wave + sub;
-  O out;
+  O out, jrow;
   P beg, end;
-  item(rr, out, beg, end);
+  item(rr, out, beg, end, jrow);
   IT c[UNROLL];
   T a[UNROLL];
-  trip(beg + slot, end, c, a);
+  QT pq[UNROLL];
+  trip(beg + slot, end, c, a, pq);
 
   for (int64_t tile = tr.first; tile < tr.end; tile += tr.step) {  // wave-uniform
     const bool active = rr < lb.count && col_ok;
     const int64_t rr_next = rr + rr_step;
     // the next item is prefetched only when this workgroup will process it
     const bool has_next = tile + tr.step < tr.end;  // wave-uniform
-    O out_n = 0;
+    O out_n = 0, jrow_n = 0;
     P beg_n = 0, end_n = 0;
-    if (has_next) item(rr_next, out_n, beg_n, end_n);
+    if (has_next) item(rr_next, out_n, beg_n, end_n, jrow_n);
+    vec_t<T, VEC> bj;
+    if constexpr (SDD) bj = b_row(jrow);
     A acc[VEC];
 #pragma unroll
     for (int v = 0; v < VEC; ++v) acc[v] = A(0);
@@ -131,7 +200,8 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
       prefetched = prefetched || !same;
       T an[UNROLL];
       IT cn[UNROLL];
-      if (same || has_next) trip(same ? pn : beg_n + slot, same ? end : end_n, cn, an);
+      QT pqn[UNROLL];
+      if (same || has_next) trip(same ? pn : beg_n + slot, same ? end : end_n, cn, an, pqn);
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
         const bool ok = p + static_cast<P>(u) * S < end;
@@ -142,13 +212,25 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
           acc[v] += av * bv;
         }
       }
+      if constexpr (SDD) {
+        // dvalues of this trip's nonzeros: a lane's VEC products, then the
+        // sum over the slot's G lanes (all of them run this trip)
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+          const A d = group_sum(col_ok ? row_dot<A>(b[u], bj) : A(0), G);
+          if (gl == 0 && p + static_cast<P>(u) * S < end) {
+            *addr<OFF32>::at(sd.dvals, static_cast<O>(pq[u]) * static_cast<O>(sizeof(T))) = static_cast<T>(d);
+          }
+        }
+      }
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
         c[u] = cn[u];
         a[u] = an[u];
+        if constexpr (SDD) pq[u] = pqn[u];
       }
     }
-    if (!prefetched && has_next) trip(beg_n + slot, end_n, c, a);
+    if (!prefetched && has_next) trip(beg_n + slot, end_n, c, a, pq);
     // Sum the S slots of a row: lanes that differ only in the slot bits.
     for (int j = 0; j < s_log2; ++j) {
 #pragma unroll
@@ -167,6 +249,7 @@ __global__ __launch_bounds__(FSP_WG_THREADS) void spmm_bins_kernel(
       }
     }
     out = out_n;
+    jrow = jrow_n;
     beg = beg_n;
     end = end_n;
     rr = rr_next;
@@ -224,18 +307,25 @@ struct spmm_args {
   int lanes_n;
   int64_t nnz;
   int64_t b_rows;
+  // the fused backward (SDD): forward B, CSR position per nonzero, dvalues
+  const void* Bj = nullptr;
+  int64_t ldbj = 0;
+  const void* pos = nullptr;
+  void* dvals = nullptr;
 };
 
-template <typename T, typename IT, typename PT, typename QT, int VEC, int UNROLL, bool PERM, bool OFF32>
+template <typename T, typename IT, typename PT, typename QT, int VEC, int UNROLL, bool PERM, bool OFF32, bool SDD>
 static void launch_bins(const spmm_args& x, dim3 grid, int g_log2, typename acc_of<T>::type* partials,
                         const launch_table& tab, hipStream_t s) {
-  hipLaunchKernelGGL((spmm_bins_kernel<T, IT, PT, QT, VEC, UNROLL, PERM, OFF32>), grid, dim3(FSP_WG_THREADS), 0, s,
-                     static_cast<const PT*>(x.indptr), static_cast<const IT*>(x.indices),
+  const sdd_args<T, QT> sd{static_cast<const T*>(x.Bj), x.ldbj, static_cast<const QT*>(x.pos),
+                           static_cast<T*>(x.dvals)};
+  hipLaunchKernelGGL((spmm_bins_kernel<T, IT, PT, QT, VEC, UNROLL, PERM, OFF32, SDD>), grid, dim3(FSP_WG_THREADS), 0,
+                     s, static_cast<const PT*>(x.indptr), static_cast<const IT*>(x.indices),
                      static_cast<const T*>(x.data), static_cast<const QT*>(x.perm), static_cast<const T*>(x.B),
-                     x.ldb, static_cast<T*>(x.C), x.ldc, x.n_dense, g_log2, partials, tab);
+                     x.ldb, static_cast<T*>(x.C), x.ldc, x.n_dense, g_log2, partials, tab, sd);
 }
 
-template <typename T, typename IT, typename PT, typename QT, int VEC, bool PERM>
+template <typename T, typename IT, typename PT, typename QT, int VEC, bool PERM, bool SDD>
 static int launch_spmm(const spmm_args& x, const fsp_csr_plan* plan, hipStream_t s) {
   using A = typename acc_of<T>::type;
   const int64_t vcols = ceil_div(x.n_dense, VEC);
@@ -258,24 +348,30 @@ static int launch_spmm(const spmm_args& x, const fsp_csr_plan* plan, hipStream_t
   A* partials = static_cast<A*>(plan->partials_dev);
   const int64_t ytiles = ceil_div(vcols, G);
   if (ytiles > 65535) return fail(FSP_ERR_ARG, "fsp_spmm_csr: n_dense %lld too large", (long long)x.n_dense);
+  if (SDD && ytiles != 1) {  // a dot needs all of a row's columns in one lane group
+    return fail(FSP_ERR_ARG, "fsp_spmm_csr_trans_sddmm: n_dense %lld exceeds one lane group (%d columns)",
+                (long long)x.n_dense, G * VEC);
+  }
   if (blocks > 0) {
     const dim3 grid(static_cast<unsigned>(blocks), static_cast<unsigned>(ytiles));
     // 32-bit byte offsets when every array the kernel addresses fits in 4 GiB.
     const int64_t lim = int64_t(1) << 32;
-    const bool off32 = std::is_same<PT, int32_t>::value && (!PERM || std::is_same<QT, int32_t>::value) &&
+    const bool off32 = std::is_same<PT, int32_t>::value &&
+                       (!(PERM || SDD) || std::is_same<QT, int32_t>::value) &&
                        x.nnz * (int64_t)sizeof(T) < lim && x.nnz * (int64_t)sizeof(IT) < lim &&
                        x.nnz * (int64_t)sizeof(QT) < lim && x.b_rows * x.ldb * (int64_t)sizeof(T) < lim &&
-                       x.n_rows * x.ldc * (int64_t)sizeof(T) < lim && partial_bytes < lim;
+                       x.n_rows * x.ldc * (int64_t)sizeof(T) < lim && partial_bytes < lim &&
+                       (!SDD || x.n_rows * x.ldbj * (int64_t)sizeof(T) < lim);
     if (off32) {
       using P32 = int32_t;
       using Q32 = int32_t;
       if (plan->unroll >= 8) {
-        launch_bins<T, IT, P32, Q32, VEC, 8, PERM, true>(x, grid, g_log2, partials, tab, s);
+        launch_bins<T, IT, P32, Q32, VEC, 8, PERM, true, SDD>(x, grid, g_log2, partials, tab, s);
       } else {
-        launch_bins<T, IT, P32, Q32, VEC, 4, PERM, true>(x, grid, g_log2, partials, tab, s);
+        launch_bins<T, IT, P32, Q32, VEC, 4, PERM, true, SDD>(x, grid, g_log2, partials, tab, s);
       }
     } else {
-      launch_bins<T, IT, PT, QT, VEC, 4, PERM, false>(x, grid, g_log2, partials, tab, s);
+      launch_bins<T, IT, PT, QT, VEC, 4, PERM, false, SDD>(x, grid, g_log2, partials, tab, s);
     }
     const int st = check_launch("spmm_bins");
     if (st) return st;
@@ -298,12 +394,14 @@ static bool wide_ok(const void* B, int64_t ldb, const void* C, int64_t ldc, int6
   return aligned && n_dense % V == 0 && ldb % V == 0 && ldc % V == 0;
 }
 
-template <typename T, typename IT, typename PT, typename QT, bool PERM>
+template <typename T, typename IT, typename PT, typename QT, bool PERM, bool SDD = false>
 static int launch_spmm_vec(spmm_args x, const fsp_csr_plan* plan, hipStream_t s) {
   constexpr int V = 16 / sizeof(T);
-  if (wide_ok<T>(x.B, x.ldb, x.C, x.ldc, x.n_dense)) return launch_spmm<T, IT, PT, QT, V, PERM>(x, plan, s);
+  const bool wide = wide_ok<T>(x.B, x.ldb, x.C, x.ldc, x.n_dense) &&
+                    (!SDD || (reinterpret_cast<uintptr_t>(x.Bj) % 16 == 0 && x.ldbj % V == 0));
+  if (wide) return launch_spmm<T, IT, PT, QT, V, PERM, SDD>(x, plan, s);
   x.lanes_n = 0;
-  return launch_spmm<T, IT, PT, QT, 1, PERM>(x, plan, s);
+  return launch_spmm<T, IT, PT, QT, 1, PERM, SDD>(x, plan, s);
 }
 
 }  // namespace fsp
@@ -404,6 +502,66 @@ extern "C" int fsp_spmm_csr(int32_t value_type, int32_t index_type, int32_t indp
                             const fsp_csr_plan* plan, void* stream) {
   return spmm_entry(value_type, index_type, indptr_type, indptr_dev, indices_dev, data_dev, perm_type, perm_dev,
                     B_dev, ldb, C_dev, ldc, n_rows, n_dense, nnz, b_rows, lanes_n, plan, stream);
+}
+
+extern "C" int fsp_spmm_csr_trans_sddmm(int32_t value_type, int32_t index_type, int32_t indptr_type,
+                                        const void* colptr_dev, const void* rowind_dev, const void* data_csc_dev,
+                                        int32_t pos_type, const void* pos_dev, const void* dC_dev, int64_t lddc,
+                                        const void* B_dev, int64_t ldb, void* dB_dev, int64_t lddb,
+                                        void* dvalues_dev, int64_t n_cols, int64_t n_dense, int64_t nnz,
+                                        int64_t n_rows, int32_t lanes_n, const fsp_csr_plan* plan, void* stream) {
+  if (n_cols < 0 || n_dense < 0 || nnz < 0 || n_rows < 0) {
+    return fail(FSP_ERR_ARG, "fsp_spmm_csr_trans_sddmm: negative size");
+  }
+  int st = check_plan(plan, n_cols);
+  if (st) return st;
+  if (colptr_dev == nullptr || (nnz > 0 && (rowind_dev == nullptr || data_csc_dev == nullptr || pos_dev == nullptr ||
+                                            dvalues_dev == nullptr || dC_dev == nullptr || B_dev == nullptr)) ||
+      (n_cols > 0 && n_dense > 0 && dB_dev == nullptr)) {
+    return fail(FSP_ERR_ARG, "fsp_spmm_csr_trans_sddmm: NULL pointer");
+  }
+  if (lddc < n_dense || ldb < n_dense || lddb < n_dense) {
+    return fail(FSP_ERR_ARG, "fsp_spmm_csr_trans_sddmm: leading dimension < n_dense");
+  }
+  const hipStream_t s = as_stream(stream);
+  if (nnz == 0 || n_dense == 0) {
+    // no nonzeros: dB = 0; no dense columns: every dvalue is an empty dot, 0
+    FSP_DISPATCH_VALUE(value_type, T, {
+      if (n_cols > 0 && n_dense > 0) {
+        int64_t g = ceil_div(n_cols * n_dense, FSP_WG_THREADS);
+        if (g > 65536) g = 65536;
+        hipLaunchKernelGGL(fill_zero_rows_kernel<T>, dim3(static_cast<unsigned>(g)), dim3(FSP_WG_THREADS), 0, s,
+                           static_cast<T*>(dB_dev), lddb, n_cols, n_dense);
+        st = check_launch("fill_zero_rows");
+        if (st) return st;
+      }
+      if (nnz > 0) {
+        int64_t g = ceil_div(nnz, FSP_WG_THREADS);
+        if (g > 65536) g = 65536;
+        hipLaunchKernelGGL(fill_zero_rows_kernel<T>, dim3(static_cast<unsigned>(g)), dim3(FSP_WG_THREADS), 0, s,
+                           static_cast<T*>(dvalues_dev), int64_t(1), nnz, int64_t(1));
+        return check_launch("fill_zero_rows");
+      }
+      return FSP_OK;
+    });
+  }
+  if (n_cols == 0) return FSP_OK;  // nnz > 0 needs columns: unreachable for a valid CSC
+  spmm_args x{colptr_dev, rowind_dev, data_csc_dev, nullptr, dC_dev, lddc, dB_dev, lddb, n_cols, n_dense, lanes_n,
+              nnz, n_rows};
+  x.Bj = B_dev;
+  x.ldbj = ldb;
+  x.pos = pos_dev;
+  x.dvals = dvalues_dev;
+  FSP_DISPATCH_VALUE(value_type, T, {
+    FSP_DISPATCH_INDEX(index_type, IT, {
+      FSP_DISPATCH_INDEX(indptr_type, PT, {
+        FSP_DISPATCH_INDEX(pos_type, QT, {
+          return launch_spmm_vec<T, IT, PT, QT, false, true>(x, plan, s);
+        });
+      });
+    });
+  });
+  return FSP_OK;
 }
 
 extern "C" int fsp_spmm_csr_noalloc(int32_t value_type, int32_t index_type, int32_t indptr_type,

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define FSP_ABI_VERSION 14
+#define FSP_ABI_VERSION 15
 #define FSP_MAX_BINS 8
 
 typedef enum fsp_value_type {
@@ -205,6 +205,32 @@ int fsp_spmm_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
                  const void* B_dev, int64_t ldb, void* C_dev, int64_t ldc,
                  int64_t n_rows, int64_t n_dense, int64_t nnz, int64_t b_rows,
                  int32_t lanes_n, const fsp_csr_plan* plan, void* stream);
+
+/* The differentiable SpMM's backward in one pass over A's CSC.  No
+ * counterpart in the reference, which has no autograd; it computes what the
+ * reference's trans SpMM (spmm_csr.py:3981, op trans) and
+ * flagsparse_sddmm_csr (sddmm_csr.py:389) compute for C = A B, from one
+ * gather of dC's rows.  With A's CSC (colptr, rowind over n_cols columns,
+ * values already in CSC order in data_csc) and pos[q] = the CSR position of
+ * CSC nonzero q (the stable CSC permutation):
+ *   dB[j, :]        = sum over q of column j: data_csc[q] * dC[rowind[q], :]
+ *   dvalues[pos[q]] = <dC[rowind[q], :], B[j, :]>
+ * dC is n_rows x n_dense (lddc); B (the forward's dense operand) and dB are
+ * n_cols x n_dense (ldb, lddb); dvalues has nnz entries in CSR order.
+ * `plan` is the plan of the CSC (long columns as segments, partials as
+ * fsp_spmm_csr).  dB is bitwise equal to fsp_spmm_csr over the same CSC and
+ * plan (with B's rows 16-byte aligned wherever dC's and dB's are); dvalues
+ * are f32 (f64 for f64) dot products in the lanes' order.
+ * n_dense must fit one lane group (64 lanes x 16 bytes: 256 f32 / 512
+ * bf16 or f16 / 128 f64 columns on 16-byte aligned rows, else 64 columns);
+ * wider operands return FSP_ERR_ARG and the caller runs the two kernels. */
+int fsp_spmm_csr_trans_sddmm(int32_t value_type, int32_t index_type, int32_t indptr_type,
+                             const void* colptr_dev, const void* rowind_dev,
+                             const void* data_csc_dev, int32_t pos_type, const void* pos_dev,
+                             const void* dC_dev, int64_t lddc, const void* B_dev, int64_t ldb,
+                             void* dB_dev, int64_t lddb, void* dvalues_dev,
+                             int64_t n_cols, int64_t n_dense, int64_t nnz, int64_t n_rows,
+                             int32_t lanes_n, const fsp_csr_plan* plan, void* stream);
 
 /* Size of the static device scratch behind fsp_spmm_csr_noalloc (per GPU). */
 #define FSP_NOALLOC_SCRATCH_BYTES (16ll << 20)

@@ -117,3 +117,56 @@ def test_rejects_mismatched_values_and_devices(device):
         fs.spmm_csr_autograd(data, B.cpu(), mat)
     with pytest.raises(ValueError, match="matrix's device"):
         fs.spmm_csr_autograd(data.cpu(), B, mat)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.bfloat16, torch.float16],
+                         ids=["f32", "f64", "bf16", "f16"])
+@pytest.mark.parametrize("index_dtype", [torch.int32, torch.int64], ids=["i32", "i64"])
+@pytest.mark.parametrize("n_dense", [1, 30, 64, 128])
+def test_fused_backward_matches_two_kernels(device, dtype, index_dtype, n_dense):
+    """dB and dvalues from one pass over the CSC: dB bitwise equal to the
+    transposed SpMM, dvalues within tolerance of the SDDMM kernel and of the
+    oracle.  A^T has long rows (columns 7, 900 and 1999 of A hold 2500
+    nonzeros: CSC segments); 30 columns miss the 16-byte path."""
+    import scipy.sparse as sp
+
+    from ._helpers import long_row_matrix
+
+    ip_t, ix_t, v_t = long_row_matrix(2000, 3000, [7, 900, 1999], 2500, 5, seed=n_dense)
+    A = sp.csr_matrix((v_t, ix_t, ip_t), shape=(2000, 3000)).T.tocsr()
+    A.sort_indices()
+    ip, ix, v = A.indptr.astype(np.int64), A.indices.astype(np.int64), A.data
+    m, n = A.shape
+    data, idx, ptr = device_csr(ip, ix, v, device, dtype, index_dtype)
+    h = fs.CsrMatrix(idx, ptr, (m, n)).handle
+    rng = np.random.default_rng(n_dense)
+    Bn, dCn = rng.standard_normal((n, n_dense)), rng.standard_normal((m, n_dense))
+    B = torch.tensor(Bn, dtype=dtype, device=device)
+    dC = torch.tensor(dCn, dtype=dtype, device=device)
+    fused = h.spmm_trans_sddmm(data, dC, B)
+    assert fused is not None
+    dB, dv = fused
+    assert torch.equal(dB, h.spmm(data, dC, transpose=True))
+    two = h.sddmm(dC, B)
+    vr, Br, dCr = np_dtype_round(v, dtype), np_dtype_round(Bn, dtype), np_dtype_round(dCn, dtype)
+    ref = oracle.sddmm(ip, ix, dCr, Br)
+    assert_rel(dv, ref, dtype, "fused dvalues")
+    assert_rel(two, ref, dtype, "sddmm dvalues")
+    assert_rel(dB, oracle.spmm(ip, ix, vr, dCr, (m, n), "trans"), dtype, "fused dB")
+
+
+def test_fused_backward_width_limit_takes_two_kernels(device):
+    ip, ix, v = _graph(500, seed=4)
+    data, idx, ptr = device_csr(ip, ix, v, device, torch.float32)
+    mat = fs.CsrMatrix(idx, ptr, (500, 500))
+    # 300 float32 columns exceed one lane group (64 lanes x 4): the two kernels run
+    assert mat.handle.spmm_trans_sddmm(data, torch.randn(500, 300, device=device),
+                                       torch.randn(500, 300, device=device)) is None
+    vals = data.clone().requires_grad_(True)
+    B = torch.randn(500, 300, device=device, requires_grad=True)
+    dCn = np.random.default_rng(4).standard_normal((500, 300))
+    fs.spmm_csr_autograd(vals, B, mat).backward(torch.tensor(dCn, dtype=torch.float32, device=device))
+    Br = B.detach().double().cpu().numpy()
+    assert_rel(vals.grad, oracle.sddmm(ip, ix, np_dtype_round(dCn, torch.float32), Br), torch.float32, "dvalues")
+    assert_rel(B.grad, oracle.spmm(ip, ix, np_dtype_round(v, torch.float32), np_dtype_round(dCn, torch.float32),
+                                   (500, 500), "trans"), torch.float32, "dB")
