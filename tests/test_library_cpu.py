@@ -102,3 +102,20 @@ def test_noalloc_spmm_rejects_oversized_partials_without_a_device():
     assert b"static scratch" in lib.fsp_last_error()
     assert lib.fsp_spmm_csr_noalloc(_lib.FSP_F32, _lib.FSP_I32, _lib.FSP_I32, None, None, None, 0, None, None, 64,
                                     None, 64, 10, 64, 100, 10, 16, None, None) == -1
+
+
+def test_fused_backward_entry_rejects_bad_arguments_without_a_device():
+    lib = _lib.load()
+    st = _lib.FspCsrPlan()
+    st.n_bins = 0
+    st.unroll = 4
+    args = [_lib.FSP_F32, _lib.FSP_I32, _lib.FSP_I32, None, None, None, _lib.FSP_I32, None, None, 8, None, 8, None, 8,
+            None]
+    # negative sizes
+    assert lib.fsp_spmm_csr_trans_sddmm(*args, -1, 8, 0, 0, 16, ctypes.byref(st), None) == -1
+    assert b"negative size" in lib.fsp_last_error()
+    # no plan
+    assert lib.fsp_spmm_csr_trans_sddmm(*args, 0, 8, 0, 0, 16, None, None) == -1
+    # nonzeros without arrays
+    assert lib.fsp_spmm_csr_trans_sddmm(*args, 0, 8, 5, 0, 16, ctypes.byref(st), None) == -1
+    assert b"NULL" in lib.fsp_last_error()
