@@ -471,16 +471,19 @@ __global__ __launch_bounds__(kLdsxThreads) void spmv_vec_kernel(
 // when the window fits kLdsxBytes, else gathers from global memory, and runs
 // the rows with the vector-chunk loop (L lanes per row).  A batch of
 // matrices maps each matrix's rows (cut into parts of bounded nonzero count)
-// to parts whose window is the matrix's column range.  REG (row_len > 0):
-// every row holds row_len nonzeros from indptr_base + r * row_len, so the
-// bounds are computed and a pass's first wait is for its chunks (see
-// spmv_vec_rows).
-template <int L, typename T, typename IT, typename PT, bool REG>
+// to parts whose window is the matrix's column range.  Row bounds (BND):
+// kBndGlobal loads them from indptr; kBndComputed (row_len > 0: every row
+// holds row_len nonzeros from indptr_base + r * row_len) computes them;
+// kBndLds reads the part's slice of indptr that the workgroup copied into
+// LDS with x.  The last two leave no global load ahead of a pass's chunk
+// loads, so a pass's first wait is for its chunks (see spmv_vec_rows).
+constexpr int kBndGlobal = 0, kBndComputed = 1, kBndLds = 2;
+template <int L, typename T, typename IT, typename PT, int BND>
 __device__ __forceinline__ void spmv_part_rows(const PT* __restrict__ indptr, const IT* __restrict__ indices,
                                                const T* __restrict__ data, const T* __restrict__ xb,
                                                int64_t xoff, T* __restrict__ y, int64_t row_begin,
                                                int64_t row_end, int64_t nnz, int64_t row_len,
-                                               int64_t indptr_base) {
+                                               int64_t indptr_base, const int64_t* __restrict__ bnd) {
   using A = typename spmv_acc_of<T>::type;
   constexpr int64_t per_wave = 64 / L;
   constexpr int64_t per_pass = per_wave * kLdsxWaves;
@@ -492,9 +495,13 @@ __device__ __forceinline__ void spmv_part_rows(const PT* __restrict__ indptr, co
     const int64_t r = base + wave * per_wave + sub;
     const bool active = r < row_end;
     int64_t b = 0, e = 0;
-    if (REG) {
+    if (BND == kBndComputed) {
       b = indptr_base + r * row_len;
       e = b + row_len;
+    } else if (BND == kBndLds) {
+      const int64_t ri = active ? r - row_begin : 0;  // inside the staged slice
+      b = bnd[ri];
+      e = bnd[ri + 1];
     } else if (active) {
       b = static_cast<int64_t>(indptr[r]);
       e = static_cast<int64_t>(indptr[r + 1]);
@@ -538,18 +545,31 @@ __global__ __launch_bounds__(kLdsxThreads) void spmv_parts_kernel(const PT* __re
   const int64_t* pt = parts + 4 * static_cast<int64_t>(blockIdx.x);
   const int64_t row_begin = pt[0], row_end = pt[1], lo = pt[2], span = pt[3] - pt[2];
   const bool fits = span >= 0 && span * static_cast<int64_t>(sizeof(T)) <= kLdsxBytes;  // uniform
+  // the part's indptr slice goes into LDS after x when both fit (uniform)
+  const int64_t boff = (span * static_cast<int64_t>(sizeof(T)) + 15) & ~int64_t(15);
+  const int64_t n_bnd = row_end - row_begin + 1;
+  const bool stage_b = fits && row_len == 0 && n_bnd >= 1 && boff + n_bnd * 8 <= kLdsxBytes;
+  int64_t* bnd = reinterpret_cast<int64_t*>(fsp_part_mem + (stage_b ? boff : 0));
   if (fits) {
     for (int64_t c = threadIdx.x; c < span; c += kLdsxThreads) xs[c] = x[lo + c];
+  }
+  if (stage_b) {
+    for (int64_t i = threadIdx.x; i < n_bnd; i += kLdsxThreads) bnd[i] = static_cast<int64_t>(indptr[row_begin + i]);
   }
   __syncthreads();
 #define FSP_PART_ROWS(LN)                                                                                   \
   if (fits && row_len > 0) {                                                                              \
-    spmv_part_rows<LN, T, IT, PT, true>(indptr, indices, data, xs, lo, y, row_begin, row_end, nnz, row_len,  \
-                                        indptr_base);                                                     \
+    spmv_part_rows<LN, T, IT, PT, kBndComputed>(indptr, indices, data, xs, lo, y, row_begin, row_end, nnz,  \
+                                                row_len, indptr_base, nullptr);                           \
+  } else if (stage_b) {                                                                                   \
+    spmv_part_rows<LN, T, IT, PT, kBndLds>(indptr, indices, data, xs, lo, y, row_begin, row_end, nnz, 0, 0, \
+                                           bnd);                                                          \
   } else if (fits) {                                                                                      \
-    spmv_part_rows<LN, T, IT, PT, false>(indptr, indices, data, xs, lo, y, row_begin, row_end, nnz, 0, 0);   \
+    spmv_part_rows<LN, T, IT, PT, kBndGlobal>(indptr, indices, data, xs, lo, y, row_begin, row_end, nnz, 0,  \
+                                              0, nullptr);                                                \
   } else {                                                                                                \
-    spmv_part_rows<LN, T, IT, PT, false>(indptr, indices, data, x, 0, y, row_begin, row_end, nnz, 0, 0);     \
+    spmv_part_rows<LN, T, IT, PT, kBndGlobal>(indptr, indices, data, x, 0, y, row_begin, row_end, nnz, 0, 0, \
+                                              nullptr);                                                   \
   }
   switch (lanes_log2) {  // uniform
     case 2: FSP_PART_ROWS(4); break;

@@ -172,7 +172,8 @@ int fsp_spmv_csr(int32_t value_type, int32_t index_type, int32_t indptr_type,
  * (row_begin, row_end, col_lo, col_hi) per part: rows [row_begin, row_end)
  * use only columns [col_lo, col_hi).  One 1024-thread workgroup per part
  * copies x[col_lo, col_hi) into LDS when it holds at most 64 KiB (else it
- * gathers from global memory) and runs the rows with `lanes` (4, 8 or 16)
+ * gathers from global memory), and the part's slice of indptr beside it when
+ * both fit, and runs the rows with `lanes` (4, 8 or 16)
  * lanes per row, reading indices and data as 4-element chunks (both aligned
  * to 4 elements).  Parts must not share rows; rows outside every part are not
  * written.  Accumulation as fsp_spmv_csr; nnz = length of indices / data.
