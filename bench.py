@@ -48,19 +48,26 @@ def spmm_bytes(nnz, m, k, n, s_v=4, s_i=4, s_p=4, s_c=4):
 
 
 def time_steps(fn, steps, warmup, stream):
-    """Warm up, then time `steps` calls between device syncs; per-step HIP
-    event durations on `stream` (the stream the kernels are launched on)."""
+    """Warm up, then time `steps` calls between device syncs (wall clock,
+    nothing else in the stream), then `steps` more calls each bracketed by
+    HIP events on `stream` (the stream the kernels are launched on).
+    Returns (wall ms of the uninstrumented steps, per-step event ms): the
+    event records cost the stream ~10 us per step, so they stay out of the
+    wall-clock window."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1000.0
+    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     for s, e in events:
         s.record(stream)
         fn()
         e.record(stream)
     torch.cuda.synchronize()
-    wall_ms = (time.perf_counter() - t0) * 1000.0
     ev = [s.elapsed_time(e) for s, e in events]
     return wall_ms, ev
 
