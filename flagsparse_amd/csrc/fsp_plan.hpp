@@ -122,7 +122,20 @@ static int64_t build_launch_table(const fsp_csr_plan* plan, int seg_lanes_log2, 
   int64_t tiles[kMaxLaunchBins];
   int64_t total_tiles = 0;
   t->n = 0;
-  for (int i = 0; i < plan->n_bins && i < FSP_MAX_BINS; ++i) {
+  // Longest work first: the long-row segments, then the bins from the
+  // longest rows down.  Workgroups dispatch in block order, so the items
+  // that take longest start first and the short rows fill in behind them.
+  if (plan->n_segments > 0) {
+    launch_bin& lb = t->b[t->n];
+    lb.rows = nullptr;
+    lb.count = plan->n_segments;
+    lb.lanes_log2 = seg_lanes_log2;
+    lb.is_segment = 1;
+    tiles[t->n] = ceil_div(plan->n_segments, items_per_block(seg_lanes_log2, 1));
+    total_tiles += tiles[t->n];
+    ++t->n;
+  }
+  for (int i = (plan->n_bins < FSP_MAX_BINS ? plan->n_bins : FSP_MAX_BINS) - 1; i >= 0; --i) {
     const fsp_bin& b = plan->bins[i];
     if (b.count <= 0) continue;
     launch_bin& lb = t->b[t->n];
@@ -133,16 +146,6 @@ static int64_t build_launch_table(const fsp_csr_plan* plan, int seg_lanes_log2, 
     lb.lanes_log2 = ll;
     lb.is_segment = 0;
     tiles[t->n] = ceil_div(b.count, items_per_block(ll, 0));
-    total_tiles += tiles[t->n];
-    ++t->n;
-  }
-  if (plan->n_segments > 0) {
-    launch_bin& lb = t->b[t->n];
-    lb.rows = nullptr;
-    lb.count = plan->n_segments;
-    lb.lanes_log2 = seg_lanes_log2;
-    lb.is_segment = 1;
-    tiles[t->n] = ceil_div(plan->n_segments, items_per_block(seg_lanes_log2, 1));
     total_tiles += tiles[t->n];
     ++t->n;
   }
