@@ -40,13 +40,120 @@ constexpr int kLdsxWaves = 16;
 constexpr int kLdsxThreads = 64 * kLdsxWaves;
 constexpr int64_t kLdsxBytes = 65536;
 
+// Vector of BYTES bytes as 32-bit words (one global / LDS access).
+template <int BYTES>
+struct chunk_words;
+template <>
+struct chunk_words<8> {
+  typedef uint32_t type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct chunk_words<16> {
+  typedef uint32_t type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct chunk_words<32> {
+  typedef uint32_t type __attribute__((ext_vector_type(8)));
+};
+
+// Unsigned integer of BYTES bytes (element-wise copies move raw words).
+template <int BYTES>
+struct word_of;
+template <>
+struct word_of<2> {
+  typedef uint16_t type;
+};
+template <>
+struct word_of<4> {
+  typedef uint32_t type;
+};
+template <>
+struct word_of<8> {
+  typedef uint64_t type;
+};
+
+// An empty asm that reads and writes v: the load producing v completes here
+// (the compiler waits for it before the asm) and cannot sink below it.
+template <typename V>
+__device__ __forceinline__ void pin(V& v) {
+  asm volatile("" : "+v"(v));
+}
+
+// dst[0, n) = src[0, n) by a 1024-thread workgroup, for n * sizeof(E) <=
+// kLdsxBytes (dst in LDS, 16-byte aligned).  Every load of a thread is issued
+// before its first LDS write, so the copy costs one memory round trip: a
+// plain strided loop waits for each load before its write, one round trip per
+// 4 KiB pass (16 for a 64 KiB window).  The loads are unconditional, with
+// indices clamped to the last element, and pinned before the guarded writes:
+// the compiler would otherwise sink a load into its write's guard, where it
+// waits for every outstanding load.  The writes stay guarded (lanes past the
+// end writing one LDS address would conflict).  A 16-byte aligned source
+// moves in 16-byte vectors (at most four per thread) plus a tail of fewer
+// than one vector; an unaligned source moves element by element, at most
+// eight loads per thread at a time.  Indices are 32-bit (n <= 32768), so an
+// address is one register over the uniform base: the copy must not raise
+// the loop kernels' registers past what two workgroups per CU allow.
+// No barrier: the caller's __syncthreads() publishes dst.
+template <typename E>
+__device__ __forceinline__ void copy_to_lds(const E* __restrict__ src, E* dst, int64_t n64) {
+  if (n64 <= 0) return;
+  const int n = static_cast<int>(n64);
+  const int t = static_cast<int>(threadIdx.x);
+  if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+    using W = typename chunk_words<16>::type;
+    constexpr int kPer = static_cast<int>(kLdsxBytes / (16 * kLdsxThreads));
+    constexpr int kElems = 16 / static_cast<int>(sizeof(E));
+    const int nv = n / kElems;
+    if (nv > 0) {
+      const W* s = reinterpret_cast<const W*>(src);
+      W* d = reinterpret_cast<W*>(dst);
+      W v[kPer];
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const int i = t + k * kLdsxThreads;
+        v[k] = s[i < nv ? i : nv - 1];
+      }
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) pin(v[k]);
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const int i = t + k * kLdsxThreads;
+        if (i < nv) d[i] = v[k];
+      }
+    }
+    const int done = nv * kElems;
+    if (t < n - done) dst[done + t] = src[done + t];  // fewer than kElems elements
+    return;
+  }
+  using U = typename word_of<sizeof(E)>::type;
+  constexpr int kPer = 8;
+  constexpr int kStep = kPer * kLdsxThreads;
+  const U* s = reinterpret_cast<const U*>(src);
+  U* d = reinterpret_cast<U*>(dst);
+  for (int base = 0; base < n; base += kStep) {  // uniform
+    U v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = base + t + k * kLdsxThreads;
+      v[k] = s[i < n ? i : n - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) pin(v[k]);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = base + t + k * kLdsxThreads;
+      if (i < n) d[i] = v[k];
+    }
+  }
+}
+
 // Pointer to x for the loop: LDS (after the workgroup's copy) or global.
 template <typename T, bool LDSX>
 __device__ __forceinline__ const T* stage_x(const T* __restrict__ x, int64_t x_len) {
   if constexpr (LDSX) {
     extern __shared__ __align__(16) unsigned char fsp_ldsx_mem[];
     T* xs = reinterpret_cast<T*>(fsp_ldsx_mem);
-    for (int64_t c = threadIdx.x; c < x_len; c += blockDim.x) xs[c] = x[c];
+    copy_to_lds(x, xs, x_len);
     __syncthreads();
     return xs;
   } else {
@@ -60,6 +167,7 @@ __global__ __launch_bounds__(64 * WAVES) void spmv_bins_kernel(
     const QT* __restrict__ perm, const T* __restrict__ x_in, T* __restrict__ y,
     typename spmv_acc_of<T>::type* __restrict__ partials, launch_table tab, int64_t x_len) {
   using A = typename spmv_acc_of<T>::type;
+  static_assert(!LDSX || WAVES == kLdsxWaves, "the LDS copy of x runs on 1024-thread workgroups");
   const T* __restrict__ x = stage_x<T, LDSX>(x_in, x_len);
   const int k = find_bin(tab, blockIdx.x);
   const launch_bin lb = tab.b[k];  // a copy: the fields stay in registers
@@ -161,6 +269,7 @@ __global__ __launch_bounds__(64 * WAVES) void spmv_simple_kernel(
     const QT* __restrict__ perm, const T* __restrict__ x_in, T* __restrict__ y,
     typename spmv_acc_of<T>::type* __restrict__ partials, launch_table tab, int64_t x_len) {
   using A = typename spmv_acc_of<T>::type;
+  static_assert(!LDSX || WAVES == kLdsxWaves, "the LDS copy of x runs on 1024-thread workgroups");
   const T* __restrict__ x = stage_x<T, LDSX>(x_in, x_len);
   const int k = find_bin(tab, blockIdx.x);
   const launch_bin lb = tab.b[k];  // a copy: the fields stay in registers
@@ -227,21 +336,6 @@ __global__ __launch_bounds__(64 * WAVES) void spmv_simple_kernel(
 template <typename E>
 struct chunk4 {
   E v[4];
-};
-
-template <int BYTES>
-struct chunk_words;
-template <>
-struct chunk_words<8> {
-  typedef uint32_t type __attribute__((ext_vector_type(2)));
-};
-template <>
-struct chunk_words<16> {
-  typedef uint32_t type __attribute__((ext_vector_type(4)));
-};
-template <>
-struct chunk_words<32> {
-  typedef uint32_t type __attribute__((ext_vector_type(8)));
 };
 
 // Four consecutive elements at p (p aligned to 4 * sizeof(E)), streamed.
@@ -533,6 +627,104 @@ __device__ __forceinline__ void spmv_part_rows(const PT* __restrict__ indptr, co
   }
 }
 
+// Parts whose row pointers are staged in LDS, whose nonzeros all lie below
+// nnz rounded down to a whole chunk and whose rows mostly fit one trip (the
+// kernel's conditions below), software-pipelined over passes: the row bounds
+// need no global load, so each pass first loads the next pass's first-trip
+// chunks and a wave keeps two passes in flight.  What keeps the
+// waits exact (the compiler's counters are in order and merge paths
+// pessimistically):
+//  * the loads are unconditional: a lane past its row's end, or a row past
+//    the part's end (which takes the part's last row), loads a chunk clamped
+//    into the arrays and masks it at the FMA;
+//  * the y store is unconditional: every lane of a group holds the group
+//    total, bitwise identical, and a clamped row stores the last row's total
+//    to the last row, the value that row's own group stores;
+//  * two register buffers alternate (a copy of a loaded register would wait
+//    for its load), and an empty asm with a memory clobber keeps the next
+//    pass's loads ahead of this pass's LDS reads.
+// Further trips of a row longer than one trip load in place.  Per lane the
+// chunks are summed in the order of spmv_part_rows (bitwise equal).
+template <typename T, typename IT>
+struct part_buf {
+  int64_t b, e;
+  chunk4<IT> c;
+  chunk4<T> v;
+};
+
+template <int L, typename T, typename IT>
+__device__ __forceinline__ void part_load(const IT* __restrict__ indices, const T* __restrict__ data,
+                                          const int64_t* __restrict__ bnd, int64_t ri, int64_t last,
+                                          int64_t max_q, part_buf<T, IT>& f) {
+  const int64_t rc = ri < last ? ri : last;
+  f.b = bnd[rc];
+  f.e = bnd[rc + 1];
+  const int64_t q = (f.b & ~int64_t(3)) + 4 * static_cast<int64_t>(threadIdx.x % L);
+  const int64_t qc = q < max_q ? q : max_q;
+  f.c = load_chunk_nt(indices + qc);
+  f.v = load_chunk_nt(data + qc);
+}
+
+template <int L, typename T, typename IT>
+__device__ __forceinline__ void part_pass(const IT* __restrict__ indices, const T* __restrict__ data,
+                                          const T* __restrict__ xb, int64_t xoff, T* __restrict__ y,
+                                          const int64_t* __restrict__ bnd, int64_t row_begin, int64_t last,
+                                          int64_t max_q, int64_t ri, int64_t ri_next, const part_buf<T, IT>& f,
+                                          part_buf<T, IT>& g) {
+  using A = typename spmv_acc_of<T>::type;
+  part_load<L>(indices, data, bnd, ri_next, last, max_q, g);
+  asm volatile("" ::: "memory");
+  const int l = threadIdx.x % L;
+  const int64_t q0 = (f.b & ~int64_t(3)) + 4 * l;
+  A acc = A(0);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const bool ok = q0 + j >= f.b && q0 + j < f.e;
+    const int64_t col = ok ? static_cast<int64_t>(f.c.v[j]) : xoff;
+    const A t = fmadd(static_cast<A>(f.v.v[j]), static_cast<A>(xb[col - xoff]), acc);
+    acc = ok ? t : acc;
+  }
+  for (int64_t q = q0 + 4 * L; q < f.e; q += 4 * L) {  // rows longer than one trip (q + 4 <= e <= max_q + 4)
+    const chunk4<IT> c2 = load_chunk_nt(indices + q);
+    const chunk4<T> v2 = load_chunk_nt(data + q);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const bool ok = q + j < f.e;
+      const int64_t col = ok ? static_cast<int64_t>(c2.v[j]) : xoff;
+      const A t = fmadd(static_cast<A>(v2.v[j]), static_cast<A>(xb[col - xoff]), acc);
+      acc = ok ? t : acc;
+    }
+  }
+  acc = group_sum(acc, L);
+  y[row_begin + (ri < last ? ri : last)] = static_cast<T>(acc);
+}
+
+template <int L, typename T, typename IT>
+__device__ __forceinline__ void spmv_part_rows_pipe(const IT* __restrict__ indices, const T* __restrict__ data,
+                                                    const T* __restrict__ xb, int64_t xoff, T* __restrict__ y,
+                                                    int64_t row_begin, int64_t row_end, int64_t max_q,
+                                                    const int64_t* __restrict__ bnd) {
+  constexpr int64_t per_wave = 64 / L;
+  constexpr int64_t per_pass = per_wave * kLdsxWaves;
+  const int64_t last = row_end - 1 - row_begin;
+  const int64_t n_pass = (last + per_pass) / per_pass;  // >= 1
+  const int64_t r0 = (threadIdx.x >> 6) * per_wave + (threadIdx.x & 63) / L;  // row of this lane in a pass
+  auto next = [n_pass](int64_t k) { return k + 1 < n_pass ? k + 1 : k; };
+  part_buf<T, IT> f0, f1;
+  part_load<L>(indices, data, bnd, r0, last, max_q, f0);
+  // pass 0 peeled: both edges into the loop then end with a y store, so the
+  // waits inside count it exactly
+  part_pass<L>(indices, data, xb, xoff, y, bnd, row_begin, last, max_q, r0, next(0) * per_pass + r0, f0, f1);
+  for (int64_t k = 1; k < n_pass;) {  // uniform over the workgroup
+    part_pass<L>(indices, data, xb, xoff, y, bnd, row_begin, last, max_q, k * per_pass + r0,
+                 next(k) * per_pass + r0, f1, f0);
+    if (++k >= n_pass) break;
+    part_pass<L>(indices, data, xb, xoff, y, bnd, row_begin, last, max_q, k * per_pass + r0,
+                 next(k) * per_pass + r0, f0, f1);
+    ++k;
+  }
+}
+
 template <typename T, typename IT, typename PT>
 __global__ __launch_bounds__(kLdsxThreads) void spmv_parts_kernel(const PT* __restrict__ indptr,
                                                                   const IT* __restrict__ indices,
@@ -550,17 +742,49 @@ __global__ __launch_bounds__(kLdsxThreads) void spmv_parts_kernel(const PT* __re
   const int64_t n_bnd = row_end - row_begin + 1;
   const bool stage_b = fits && row_len == 0 && n_bnd >= 1 && boff + n_bnd * 8 <= kLdsxBytes;
   int64_t* bnd = reinterpret_cast<int64_t*>(fsp_part_mem + (stage_b ? boff : 0));
-  if (fits) {
-    for (int64_t c = threadIdx.x; c < span; c += kLdsxThreads) xs[c] = x[lo + c];
-  }
-  if (stage_b) {
-    for (int64_t i = threadIdx.x; i < n_bnd; i += kLdsxThreads) bnd[i] = static_cast<int64_t>(indptr[row_begin + i]);
+  if (fits) copy_to_lds(x + lo, xs, span);
+  if (stage_b) {  // n_bnd * 8 <= kLdsxBytes: at most kPer entries per thread (loads as in copy_to_lds)
+    constexpr int kPer = static_cast<int>(kLdsxBytes / (8 * kLdsxThreads));
+    const PT* src = indptr + row_begin;
+    const int nb = static_cast<int>(n_bnd);  // <= 8192: 32-bit indices (see copy_to_lds)
+    PT v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = static_cast<int>(threadIdx.x) + k * kLdsxThreads;
+      v[k] = src[i < nb ? i : nb - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) pin(v[k]);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = static_cast<int>(threadIdx.x) + k * kLdsxThreads;
+      if (i < nb) bnd[i] = static_cast<int64_t>(v[k]);
+    }
   }
   __syncthreads();
+  // The pipelined loop (uniform conditions): row pointers in LDS, every
+  // nonzero of the part inside the whole chunks of the arrays (the clamped
+  // loads stay in bounds), and a mean row length + 3 of at most 4 L, one
+  // trip (a row longer than one trip loads its further trips in place, and
+  // then waits for the next pass's loads as well: at 2-3 trips a row the
+  // plain loop measured faster).  8-byte values with 8-byte column ids keep
+  // the plain loop: the two buffers would take the kernel past 64 vector
+  // registers.  Constant-length rows keep their computed-bounds loop: a
+  // pipelined form of it gained nothing and its paths raised this kernel's
+  // scalar registers past what two 16-wave workgroups per CU allow (84
+  // against 72), which slowed every part.
+  const int64_t max_q = (nnz & ~int64_t(3)) - 4;
+  const int64_t n_rows_p = row_end - row_begin;
+  constexpr bool kPipeTypes = sizeof(T) + sizeof(IT) <= 12;
+  const bool pipe_ok = kPipeTypes && stage_b && n_rows_p > 0 && max_q >= 0 && bnd[0] >= 0 &&
+                       bnd[n_bnd - 1] <= max_q + 4;
+  const int64_t trip_nnz = pipe_ok ? (bnd[n_bnd - 1] - bnd[0]) + 3 * n_rows_p : 0;
 #define FSP_PART_ROWS(LN)                                                                                   \
   if (fits && row_len > 0) {                                                                              \
     spmv_part_rows<LN, T, IT, PT, kBndComputed>(indptr, indices, data, xs, lo, y, row_begin, row_end, nnz,  \
                                                 row_len, indptr_base, nullptr);                           \
+  } else if (pipe_ok && trip_nnz <= 4 * (LN) * n_rows_p) {                                                \
+    spmv_part_rows_pipe<LN, T, IT>(indices, data, xs, lo, y, row_begin, row_end, max_q, bnd);              \
   } else if (stage_b) {                                                                                   \
     spmv_part_rows<LN, T, IT, PT, kBndLds>(indptr, indices, data, xs, lo, y, row_begin, row_end, nnz, 0, 0, \
                                            bnd);                                                          \

@@ -406,6 +406,37 @@ def test_per_matrix_loops_take_strided_operands(device):
         b.spmm(Bs.contiguous(), out=torch.empty(b.total_rows, 13, device=device))
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.bfloat16], ids=["f32", "f64", "bf16"])
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+def test_pipelined_parts_match_plain_loop(device, dtype, lanes):
+    """Small parts (row pointers staged in LDS beside x) run the
+    software-pipelined parts loop where their rows fit one trip; a part
+    holding a whole 7001^2 matrix cannot stage its row pointers (8 bytes a
+    row beside x exceed 64 KiB) and runs the plain loop with bounds read
+    from HBM.  Both add each lane's chunks in the same order: bitwise equal.
+    The odd size puts matrices 1-3's x windows off 16-byte boundaries (the
+    element-wise LDS copy), and the oracle pins the first matrix."""
+    from flagsparse_amd.autotune import TuneParams
+
+    ip, ix, v, ro, no = S.csr_batch(n_mats=4, rows_lo=7001, rows_hi=7001, nnz_lo=4, nnz_hi=24, seed=11,
+                                    device=device)
+    b = CsrBatch(v.to(dtype), ix, ip, ro, None, no)
+    assert any(int(c) % 4 for c in b.col_offsets[:-1])  # at least one unaligned window
+    x = (torch.rand(b.total_cols, device=device) * 2 - 1).to(dtype)
+    small = TuneParams(unroll=4, lanes_n=lanes, variant=5, part_nnz=2048)
+    whole = TuneParams(unroll=4, lanes_n=lanes, variant=5, part_nnz=1 << 40)
+    assert b.parts(whole.part_nnz)[0].shape[0] == b.n_mats
+    y = b.spmv(x, params=small)
+    assert torch.equal(y, b.spmv(x, params=whole))
+    m0 = int(b.row_offsets[1])
+    n0 = int(b.col_offsets[1])
+    ip0 = ip[: m0 + 1].cpu().numpy().astype(np.int64)
+    nz0 = int(ip0[-1])
+    vr = b.data[:nz0].double().cpu().numpy()
+    ref = O.spmv(ip0, ix[:nz0].cpu().numpy(), vr, x[:n0].double().cpu().numpy(), (m0, n0))
+    assert rel_err(y[:m0], ref) <= REL_TOL[dtype]
+
+
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
 @pytest.mark.parametrize("lanes", [4, 8, 16])
 def test_constant_row_length_batch_computes_bounds(device, dtype, lanes):
