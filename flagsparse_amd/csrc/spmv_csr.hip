@@ -451,10 +451,23 @@ __device__ __forceinline__ void spmv_vec_rows(const launch_bin& lb, const launch
 // 4 * L * CHUNKS, every row inside the arrays): software-pipelined over row
 // tiles.  The next tile's chunks load before this tile's are summed, from
 // addresses that need no load, so a wave keeps one tile in flight while it
-// works on another.  The loads are unconditional (a finished or short lane
-// reloads a chunk it already holds, and masks it at the FMA): a load under a
-// branch would make the compiler wait for everything outstanding at the join.
-// Per lane the terms are added in the order of spmv_vec_rows (bitwise equal).
+// works on another.  As in the pipelined parts loop, the waits come out
+// exact only because (1) the loads are unconditional (a finished or short
+// lane reloads a chunk it already holds, a row past the bin's end reloads
+// the last row, and the FMA masks them); (2) the y store is unconditional
+// (every lane of a group holds the group total, bitwise identical, and a row
+// past the end stores the last row's own total to the last row); (3) two
+// register buffers alternate, with the first tile peeled, so no loaded
+// register is copied and every edge into the loop ends with a store; an
+// empty asm with a memory clobber keeps the next tile's loads ahead of this
+// tile's x reads.  Per lane the terms are added in the order of
+// spmv_vec_rows (bitwise equal).
+template <typename T, typename IT, int CHUNKS>
+struct vec_tile {
+  chunk4<IT> c[CHUNKS];
+  chunk4<T> v[CHUNKS];
+};
+
 template <int L, typename T, typename IT, int CHUNKS>
 __device__ __forceinline__ void spmv_vec_rows_pipe(const launch_bin& lb, const launch_table& tab,
                                                    const IT* __restrict__ indices, const T* __restrict__ data,
@@ -469,54 +482,54 @@ __device__ __forceinline__ void spmv_vec_rows_pipe(const launch_bin& lb, const l
   const int64_t len = tab.row_len;
   const int64_t last = lb.count - 1;
   const tile_range tr = bin_tiles(lb, per_block);
+  if (tr.first >= tr.end) return;  // uniform over the workgroup
   int64_t off[CHUNKS];  // this lane's chunk offsets inside a row (clamped to 0 past its end)
 #pragma unroll
   for (int u = 0; u < CHUNKS; ++u) {
     const int64_t o = 4 * l + 4 * L * u;
     off[u] = o < len ? o : 0;
   }
-  auto row_base = [&](int64_t tile) {
+  auto row_of = [&](int64_t tile) {
     const int64_t rr = tile * per_block + wave * per_wave + sub;
-    return tab.indptr_base + (rr < last ? rr : last) * len;
+    return rr < last ? rr : last;
   };
-  chunk4<IT> c[CHUNKS];
-  chunk4<T> v[CHUNKS];
-  if (tr.first < tr.end) {
-    const int64_t b = row_base(tr.first);
+  auto load = [&](int64_t tile, vec_tile<T, IT, CHUNKS>& f) {
+    const int64_t b = tab.indptr_base + row_of(tile) * len;
 #pragma unroll
     for (int u = 0; u < CHUNKS; ++u) {
-      c[u] = load_chunk_nt(indices + b + off[u]);
-      v[u] = load_chunk_nt(data + b + off[u]);
+      f.c[u] = load_chunk_nt(indices + b + off[u]);
+      f.v[u] = load_chunk_nt(data + b + off[u]);
     }
-  }
-  for (int64_t tile = tr.first; tile < tr.end; tile += tr.step) {
-    const int64_t next = tile + tr.step < tr.end ? tile + tr.step : tile;
-    const int64_t bn = row_base(next);
-    chunk4<IT> cn[CHUNKS];
-    chunk4<T> vn[CHUNKS];
-#pragma unroll
-    for (int u = 0; u < CHUNKS; ++u) {
-      cn[u] = load_chunk_nt(indices + bn + off[u]);
-      vn[u] = load_chunk_nt(data + bn + off[u]);
-    }
-    const int64_t rr = tile * per_block + wave * per_wave + sub;
+  };
+  // sums tile `tile` from f after loading tile `next` into g
+  auto pass = [&](int64_t tile, int64_t next, const vec_tile<T, IT, CHUNKS>& f, vec_tile<T, IT, CHUNKS>& g) {
+    load(next, g);
+    asm volatile("" ::: "memory");
     A acc = A(0);
 #pragma unroll
     for (int u = 0; u < CHUNKS; ++u) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const bool ok = 4 * l + 4 * L * u + j < len;
-        const A t = fmadd(static_cast<A>(v[u].v[j]), static_cast<A>(x[static_cast<int64_t>(c[u].v[j])]), acc);
+        const A t = fmadd(static_cast<A>(f.v[u].v[j]), static_cast<A>(x[static_cast<int64_t>(f.c[u].v[j])]), acc);
         acc = ok ? t : acc;
       }
     }
     acc = group_sum(acc, L);
-    if (rr <= last && l == 0) y[rr] = static_cast<T>(acc);
-#pragma unroll
-    for (int u = 0; u < CHUNKS; ++u) {
-      c[u] = cn[u];
-      v[u] = vn[u];
-    }
+    y[row_of(tile)] = static_cast<T>(acc);
+  };
+  auto after = [&](int64_t tile) { return tile + tr.step < tr.end ? tile + tr.step : tile; };
+  vec_tile<T, IT, CHUNKS> f0, f1;
+  load(tr.first, f0);
+  int64_t tile = tr.first;
+  pass(tile, after(tile), f0, f1);  // peeled
+  tile += tr.step;
+  while (tile < tr.end) {  // uniform over the workgroup
+    pass(tile, after(tile), f1, f0);
+    tile += tr.step;
+    if (tile >= tr.end) break;
+    pass(tile, after(tile), f0, f1);
+    tile += tr.step;
   }
 }
 
