@@ -109,7 +109,7 @@ def batch_spmv_candidates(elem_bytes=4, x_len=0, chunk_ok=False):
     out = candidates("spmv", 1, elem_bytes, x_len, chunk_ok)
     if chunk_ok:
         for lanes in (4, 8, 16):
-            for part_nnz in (1 << 13, 1 << 15, 1 << 17):
+            for part_nnz in (1 << 13, 1 << 15, 1 << 16, 1 << 17):
                 out.append(TuneParams(unroll=4, lanes_n=lanes, variant=5, part_nnz=part_nnz))
     return out
 
